@@ -1,0 +1,300 @@
+// C ABI and host orchestration of the PE tracer.
+//
+// Replaces PESysMatGen (PEGen_RayTracing_CircularHole/PESysMatGen.cu:766-833):
+// the reference re-allocates and re-uploads fixed 80000-float parameter blocks
+// for every Z-slab; here the geometry is repacked once into HBM-resident
+// tables (tables.hpp) and slabs / detector blocks are launched against them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "../../include/sysmat.h"
+#include "common.hpp"
+#include "pe_kernel.hpp"
+
+namespace sysmat {
+
+struct PePlanImpl {
+    int device = 0;
+    Tables T;
+    DeviceBuf<Crystal> cry;
+    DeviceBuf<Hole> hol;
+    DeviceBuf<Plate> pla;
+    DeviceBuf<Slab> slb;
+    DeviceBuf<int> status;
+    PeParams base{};
+    hipStream_t last_stream = nullptr;
+    int tile_fixed = 0;
+};
+
+static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, PeParams& p) {
+    const int X = (int)std::floor(img[0] + 0.001f), Y = (int)std::floor(img[1] + 0.001f);
+    const int Z = (int)std::floor(img[2] + 0.001f);
+    if (X <= 0 || Y <= 0 || Z <= 0) return set_error(SM_E_ARG, "image voxel counts must be positive");
+    if (o.voxel_sub < 1 || o.sub_x < 1 || o.sub_y < 1 || o.sub_z < 1)
+        return set_error(SM_E_ARG, "sampling counts must be >= 1");
+    const int zb = o.z_begin, zc = o.z_count > 0 ? o.z_count : Z - o.z_begin;
+    if (zb < 0 || zc <= 0 || zb + zc > Z) return set_error(SM_E_ARG, "z range outside the image");
+    p = PeParams{};
+    p.X = X;
+    p.Y = Y;
+    p.Zslab = zc;
+    p.Zfull = Z;
+    p.z0 = zb;
+    p.wx = img[3];
+    p.wy = img[4];
+    p.wz = img[5];
+    p.sx = img[8];
+    p.sy = img[9];
+    p.sz = img[10];
+    const float ang = (float)o.rotation * img[7];
+    p.rc = std::cos(ang);
+    p.rs = std::sin(ang);
+    const double ang64 = (double)o.rotation * (double)img[7];
+    p.rc64 = std::cos(ang64);
+    p.rs64 = std::sin(ang64);
+    p.fov2c = img[11];
+    p.nv = o.voxel_sub;
+    p.cx = o.sub_x;
+    p.cy = o.sub_y;
+    p.cz = o.sub_z;
+    long long S = (long long)p.nv * p.nv * p.nv * p.cx * p.cy * p.cz;
+    if (S > (1 << 24)) return set_error(SM_E_ARG, "too many sub-rays per element");
+    p.S = (int)S;
+    p.n_det = T.n_det;
+    p.n_layers = T.n_layers;
+    p.n_holes = T.n_holes;
+    p.n_slabs = (int)T.slabs.size();
+    p.use_bundle = (o.bundle_filter && p.n_slabs > 0) ? 1 : 0;
+    p.inv4pi = (float)(1.0 / (4.0 * M_PI));
+    return SM_OK;
+}
+
+static void set_tiles(PeParams& p, int tx, int ty, int tz) {
+    p.tx = std::max(1, std::min(tx, p.X));
+    p.ty = std::max(1, std::min(ty, p.Y));
+    p.tz = std::max(1, std::min(tz, p.Zslab));
+    p.ntx = (p.X + p.tx - 1) / p.tx;
+    p.nty = (p.Y + p.ty - 1) / p.ty;
+    p.ntz = (p.Zslab + p.tz - 1) / p.tz;
+}
+
+static int upload(PePlanImpl& P, const float* col, const float* det, const float* img) {
+    std::string err = build_tables(col, det, img[11], P.T);
+    if (!err.empty()) return set_error(SM_E_ARG, err);
+    int rc;
+    if ((rc = P.cry.upload(P.T.crystals)) < 0) return rc;
+    if ((rc = P.hol.upload(P.T.holes)) < 0) return rc;
+    if ((rc = P.pla.upload(P.T.plates)) < 0) return rc;
+    if ((rc = P.slb.upload(P.T.slabs)) < 0) return rc;
+    if ((rc = P.status.alloc(1)) < 0) return rc;
+    return SM_OK;
+}
+
+// Launch with the plan's tile; on cull-list overflow halve the tile and
+// relaunch (synchronous).  Returns SM_OK or an error.
+static int run_sync(PePlanImpl& P, PeParams p, float* dst, hipStream_t stream) {
+    int t[3] = {p.tx, p.ty, p.tz};
+    for (;;) {
+        set_tiles(p, t[0], t[1], t[2]);
+        HIP_TRY(hipMemsetAsync(P.status.ptr, 0, sizeof(int), stream));
+        HIP_TRY(launch_pe(p, P.cry.ptr, P.hol.ptr, P.pla.ptr, P.slb.ptr, dst, P.status.ptr, stream));
+        int st = 0;
+        HIP_TRY(hipMemcpyAsync(&st, P.status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        if (!(st & 1)) return SM_OK;
+        if (t[0] == 1 && t[1] == 1 && t[2] == 1)
+            return set_error(SM_E_OVERFLOW, "cull list exceeds 4096 crystals / 1024 holes for a single voxel");
+        for (int& v : t) v = std::max(1, v / 2);
+    }
+}
+
+static void auto_tiles(const SmPeOptions& o, int& tx, int& ty, int& tz) {
+    tx = o.tile_x > 0 ? o.tile_x : 4;
+    ty = o.tile_y > 0 ? o.tile_y : 4;
+    tz = o.tile_z > 0 ? o.tile_z : 4;
+}
+
+}  // namespace sysmat
+
+using namespace sysmat;
+
+extern "C" {
+
+int sm_pe_default_options(SmPeOptions* o) {
+    if (!o) return set_error(SM_E_ARG, "null options");
+    std::memset(o, 0, sizeof(*o));
+    o->voxel_sub = 1;
+    o->sub_x = 8;
+    o->sub_y = 16;
+    o->sub_z = 8;
+    o->det_begin = 0;
+    o->det_count = 0;
+    o->z_begin = 0;
+    o->z_count = 0;
+    o->rotation = 0;
+    o->bundle_filter = 1;
+    o->device = 0;
+    return SM_OK;
+}
+
+int sm_pe_plan_create(const float* col, const float* det, const float* img, const SmPeOptions* o, SmPePlan** plan) {
+    clear_error();
+    if (!col || !det || !img || !o || !plan) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(o->device);
+    if (rc < 0) return rc;
+    auto* P = new PePlanImpl();
+    P->device = o->device;
+    rc = upload(*P, col, det, img);
+    if (rc >= 0) rc = fill_params(img, *o, P->T, P->base);
+    if (rc < 0) {
+        delete P;
+        return rc;
+    }
+    int tx, ty, tz;
+    auto_tiles(*o, tx, ty, tz);
+    set_tiles(P->base, tx, ty, tz);
+    P->base.det_begin = o->det_begin;
+    P->base.det_count = o->det_count > 0 ? o->det_count : P->T.n_det - o->det_begin;
+    *plan = reinterpret_cast<SmPePlan*>(P);
+    return SM_OK;
+}
+
+int sm_pe_plan_launch(SmPePlan* plan, int det_begin, int det_count, float* dst_dev, long long row_stride,
+                      void* stream) {
+    clear_error();
+    auto* P = reinterpret_cast<PePlanImpl*>(plan);
+    if (!P || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    if (det_begin < 0 || det_count <= 0 || det_begin + det_count > P->T.n_det)
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    int rc = use_device(P->device);
+    if (rc < 0) return rc;
+    PeParams p = P->base;
+    p.det_begin = det_begin;
+    p.det_count = det_count;
+    p.row_stride = row_stride > 0 ? row_stride : (long long)p.X * p.Y * p.Zslab;
+    hipStream_t s = (hipStream_t)stream;
+    P->last_stream = s;
+    HIP_TRY(launch_pe(p, P->cry.ptr, P->hol.ptr, P->pla.ptr, P->slb.ptr, dst_dev, P->status.ptr, s));
+    return SM_OK;
+}
+
+int sm_pe_plan_sync(SmPePlan* plan) {
+    clear_error();
+    auto* P = reinterpret_cast<PePlanImpl*>(plan);
+    if (!P) return set_error(SM_E_ARG, "null plan");
+    int rc = use_device(P->device);
+    if (rc < 0) return rc;
+    int st = 0;
+    HIP_TRY(hipStreamSynchronize(P->last_stream));
+    HIP_TRY(hipMemcpy(&st, P->status.ptr, sizeof(int), hipMemcpyDeviceToHost));
+    if (st & 1) return set_error(SM_E_OVERFLOW, "cull list overflow in a tile: use smaller tiles");
+    return SM_OK;
+}
+
+int sm_pe_plan_destroy(SmPePlan* plan) {
+    auto* P = reinterpret_cast<PePlanImpl*>(plan);
+    if (P) {
+        use_device(P->device);
+        delete P;
+    }
+    return SM_OK;
+}
+
+long long sm_pe_plan_subrays(const SmPePlan* plan) {
+    auto* P = reinterpret_cast<const PePlanImpl*>(plan);
+    return P ? (long long)P->base.S : 0;
+}
+
+int sm_pe_compute_device(const float* col, const float* det, const float* img, const SmPeOptions* o,
+                         float* dst_dev, long long row_stride, void* stream) {
+    SmPePlan* plan = nullptr;
+    int rc = sm_pe_plan_create(col, det, img, o, &plan);
+    if (rc < 0) return rc;
+    auto* P = reinterpret_cast<PePlanImpl*>(plan);
+    PeParams p = P->base;
+    p.row_stride = row_stride > 0 ? row_stride : (long long)p.X * p.Y * p.Zslab;
+    if (p.det_begin < 0 || p.det_count <= 0 || p.det_begin + p.det_count > P->T.n_det) {
+        sm_pe_plan_destroy(plan);
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    }
+    rc = run_sync(*P, p, dst_dev, (hipStream_t)stream);
+    sm_pe_plan_destroy(plan);
+    return rc;
+}
+
+int sm_pe_compute(const float* col, const float* det, const float* img, const SmPeOptions* o, float* dst) {
+    clear_error();
+    if (!dst || !o || !img) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(o->device);
+    if (rc < 0) return rc;
+    const int X = (int)std::floor(img[0] + 0.001f), Y = (int)std::floor(img[1] + 0.001f);
+    const int Z = (int)std::floor(img[2] + 0.001f);
+    const int N = (int)std::floor(det[0] + 0.000001f);
+    const int zc = o->z_count > 0 ? o->z_count : Z - o->z_begin;
+    const int dc = o->det_count > 0 ? o->det_count : N - o->det_begin;
+    if (X <= 0 || Y <= 0 || zc <= 0 || dc <= 0) return set_error(SM_E_ARG, "empty output range");
+    const size_t n = (size_t)dc * X * Y * zc;
+    DeviceBuf<float> out;
+    if ((rc = out.alloc(n)) < 0) return rc;
+    rc = sm_pe_compute_device(col, det, img, o, out.ptr, 0, nullptr);
+    if (rc < 0) return rc;
+    HIP_TRY(hipMemcpy(dst, out.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+    return SM_OK;
+}
+
+int sm_PESysMatGen(const float* col, const float* det, const float* img, float* dst, int cuda_id) {
+    clear_error();
+    if (!col || !det || !img || !dst) return set_error(SM_E_ARG, "null argument");
+    // in-band slab description (PEGen_CircularHole.cpp:165,177,189-190)
+    const int X = (int)std::floor(img[0]), Y = (int)std::floor(img[1]);
+    const int slabZ = (int)std::floor(img[2]);
+    const int fullZ = (int)std::floor(img[30]);
+    const int z0 = (int)std::floor(img[31] + 0.001f);
+    if (X <= 0 || Y <= 0 || slabZ <= 0) return set_error(SM_E_ARG, "image voxel counts must be positive");
+    SmPeOptions o;
+    sm_pe_default_options(&o);
+    o.device = cuda_id;
+    o.rotation = (int)std::floor(img[20] + 0.001f);
+    // The kernel positions voxels with the full Z (Img[30]) and computes the
+    // slab's planes z0 .. z0+slabZ-1; present that as a full image of fullZ.
+    float img2[32];
+    std::memcpy(img2, img, sizeof(img2));
+    img2[2] = (float)std::max(fullZ, z0 + slabZ);
+    o.z_begin = z0;
+    o.z_count = slabZ;
+    // The reference uses Img[30] for voxel positions; keep it when it differs
+    // from the slab range (fullZ == 0 means "unset" in the reference too).
+    SmPePlan* plan = nullptr;
+    int rc = sm_pe_plan_create(col, det, img2, &o, &plan);
+    if (rc < 0) return rc;
+    auto* P = reinterpret_cast<PePlanImpl*>(plan);
+    P->base.Zfull = fullZ;
+    const size_t n = (size_t)P->base.det_count * X * Y * slabZ;
+    DeviceBuf<float> out;
+    if ((rc = out.alloc(n)) < 0) {
+        sm_pe_plan_destroy(plan);
+        return rc;
+    }
+    PeParams p = P->base;
+    p.row_stride = (long long)X * Y * slabZ;
+    rc = run_sync(*P, p, out.ptr, nullptr);
+    sm_pe_plan_destroy(plan);
+    if (rc < 0) return rc;
+    HIP_TRY(hipMemcpy(dst, out.ptr, n * sizeof(float), hipMemcpyDeviceToHost));
+    return X * Y * slabZ;
+}
+
+}  // extern "C"
+
+// C++-linkage drop-in with the reference's exact signature (PESysMatGen.h:4-9).
+int PESysMatGen(float* parameter_Collimator, float* parameter_Detector, float* parameter_Image, float* dst,
+                int cuda_id) {
+    int rc = sm_PESysMatGen(parameter_Collimator, parameter_Detector, parameter_Image, dst, cuda_id);
+    if (rc < 0) std::fprintf(stderr, "PESysMatGen: %s\n", sm_last_error());
+    return rc;
+}

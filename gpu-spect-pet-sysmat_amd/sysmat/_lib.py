@@ -1,0 +1,89 @@
+"""ctypes binding of libsysmat_hip.so (the C ABI declared in include/sysmat.h).
+
+The library is loaded from the package directory (built in-tree by
+``__graft_entry__.build()`` / ``make``).  Loading fails loudly: there is no CPU
+fallback on the product path.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB_PATH = os.path.join(PKG_DIR, "libsysmat_hip.so")
+
+SM_OK = 0
+SM_E_ARG = -1
+SM_E_DEVICE = -2
+SM_E_NOMEM = -3
+SM_E_OVERFLOW = -4
+SM_E_IO = -5
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+
+
+class SmPeOptions(ctypes.Structure):
+    _fields_ = [("voxel_sub", ctypes.c_int), ("sub_x", ctypes.c_int), ("sub_y", ctypes.c_int),
+                ("sub_z", ctypes.c_int), ("det_begin", ctypes.c_int), ("det_count", ctypes.c_int),
+                ("z_begin", ctypes.c_int), ("z_count", ctypes.c_int), ("rotation", ctypes.c_int),
+                ("tile_x", ctypes.c_int), ("tile_y", ctypes.c_int), ("tile_z", ctypes.c_int),
+                ("bundle_filter", ctypes.c_int), ("device", ctypes.c_int)]
+
+
+class SysmatError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"sysmat error {code}: {msg}")
+        self.code = code
+
+
+# every symbol the C header declares, with (restype, argtypes)
+_VP = ctypes.c_void_p
+SIGNATURES = {
+    "sm_version": (ctypes.c_int, []),
+    "sm_last_error": (ctypes.c_char_p, []),
+    "sm_device_count": (ctypes.c_int, []),
+    "sm_pe_default_options": (ctypes.c_int, [ctypes.POINTER(SmPeOptions)]),
+    "sm_PESysMatGen": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int]),
+    "sm_pe_compute": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), _VP]),
+    "sm_pe_compute_device": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), _VP, ctypes.c_longlong,
+                                            _VP]),
+    "sm_pe_plan_create": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), ctypes.POINTER(_VP)]),
+    "sm_pe_plan_launch": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_pe_plan_sync": (ctypes.c_int, [_VP]),
+    "sm_pe_plan_destroy": (ctypes.c_int, [_VP]),
+    "sm_pe_plan_subrays": (ctypes.c_longlong, [_VP]),
+}
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load libsysmat_hip.so and bind every declared entry point (raises if absent)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise SysmatError(SM_E_DEVICE, f"{path} is not built; run __graft_entry__.build() (make -C {PKG_DIR})")
+    L = ctypes.CDLL(path)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def check(rc: int) -> int:
+    if rc < 0:
+        msg = load().sm_last_error().decode(errors="replace")
+        raise SysmatError(rc, msg)
+    return rc
+
+
+def ptr(a) -> int:
+    """Raw address of a numpy array or torch tensor."""
+    if isinstance(a, np.ndarray):
+        return a.ctypes.data
+    return int(a.data_ptr())

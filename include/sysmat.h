@@ -1,0 +1,91 @@
+/* sysmat.h -- C ABI of libsysmat_hip.so, the MI355X (gfx950) system-matrix
+ * generator.  Plain pointers and sizes only; every entry point returns a
+ * status (>= 0 success, < 0 an SM_E* code; sm_last_error() gives the text).
+ *
+ * Reference interfaces replaced (Tridev-github/gpu-spect-pet-sysmat):
+ *   PESysMatGen(...)          PEGen_RayTracing_CircularHole/PESysMatGen.h:4-9,
+ *                             PESysMatGen.cu:766-833
+ *   scatter(...) combined     ScatterGen_RayTracing_CircularHole/scatter.h:4-5,
+ *                             scatter.cu:2013-2378
+ *   scatter(...) collimator   ScatterGen_Collimator/scatter.h:4-5, scatter.cu:1511-1747
+ *   scatter(...) crystal      ScatterGen_Crystal/scatter.h:4, scatter.cu:1528-1738
+ * The C++-linkage originals are also exported (see sysmat_ref.h) so the
+ * reference driver sources link against this library unchanged.
+ *
+ * Parameter arrays are the reference's raw float32 Params_*.dat contents
+ * (layout: PEGen_RayTracing_CircularHole/ReadMe.txt).  Counts are read from
+ * the headers (Col[0], Col[(m+1)*10], Det[0]); buffers must hold at least
+ * 100 + 9*holes (Col), 1 + 12*N (Det) and 32 (Img) floats.
+ */
+#ifndef SYSMAT_H_
+#define SYSMAT_H_
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SM_OK 0
+#define SM_E_ARG (-1)       /* invalid argument / parameter array */
+#define SM_E_DEVICE (-2)    /* HIP device missing or call failed */
+#define SM_E_NOMEM (-3)     /* device allocation failed */
+#define SM_E_OVERFLOW (-4)  /* cull list overflow even with 1x1x1 tiles */
+#define SM_E_IO (-5)        /* file read / write failed */
+
+/* Library version (major*10000 + minor*100 + patch). */
+int sm_version(void);
+/* Text of the last error on this thread ("" if none). */
+const char* sm_last_error(void);
+/* Number of visible HIP devices (0 when none). */
+int sm_device_count(void);
+
+/* ---- photoelectric system matrix -------------------------------------- */
+
+typedef struct SmPeOptions {
+    int voxel_sub;             /* voxel sub-points per axis, averaged (1 = voxel centre, reference) */
+    int sub_x, sub_y, sub_z;   /* crystal sub-cells, summed (8, 16, 8 = reference, PESysMatGen.cu:435) */
+    int det_begin, det_count;  /* detector rows to compute (det_count <= 0: all from det_begin) */
+    int z_begin, z_count;      /* voxel z range of the slab (z_count <= 0: to the end) */
+    int rotation;              /* rotation index (reference: Img[20]) */
+    int tile_x, tile_y, tile_z;/* voxel tile per workgroup (0 = automatic) */
+    int bundle_filter;         /* 1 = skip attenuators no sub-ray of an element can touch */
+    int device;                /* HIP device id */
+} SmPeOptions;
+
+/* Reference defaults: voxel centre, 8x16x8 crystal sub-cells, all detectors,
+ * whole volume, rotation 0, automatic tiles, filter on, device 0. */
+int sm_pe_default_options(SmPeOptions* o);
+
+/* Drop-in for PESysMatGen (PESysMatGen.cu:766-833): one Z-slab described
+ * in-band by Img[2] (slab Z), Img[30] (full Z), Img[31] (z0), Img[20]
+ * (rotation); dst (host) receives N x (X*Y*Img[2]) floats, det-major.
+ * Returns X*Y*Img[2] (numImagebin) like the reference, or an SM_E* code. */
+int sm_PESysMatGen(const float* col, const float* det, const float* img, float* dst, int cuda_id);
+
+/* PE rows [det_begin, det_begin+det_count) x voxels [z_begin, z_begin+z_count)
+ * into host memory dst (det_count x X*Y*z_count floats, det-major).
+ * Img[0..11] are read as in Params_Image.dat; the slab fields are taken from o. */
+int sm_pe_compute(const float* col, const float* det, const float* img, const SmPeOptions* o, float* dst);
+
+/* Same into device memory dst_dev on HIP stream `stream` (NULL = default);
+ * blocks until done.  row_stride = floats between detector rows of dst_dev
+ * (0 = X*Y*z_count). */
+int sm_pe_compute_device(const float* col, const float* det, const float* img, const SmPeOptions* o,
+                         float* dst_dev, long long row_stride, void* stream);
+
+/* Plans keep the geometry tables resident in HBM for repeated launches. */
+typedef struct SmPePlan SmPePlan;
+int sm_pe_plan_create(const float* col, const float* det, const float* img, const SmPeOptions* o, SmPePlan** plan);
+/* Asynchronous launch of detector rows [det_begin, det_begin+det_count) of the
+ * plan into dst_dev (row stride as above) on `stream`. */
+int sm_pe_plan_launch(SmPePlan* plan, int det_begin, int det_count, float* dst_dev, long long row_stride,
+                      void* stream);
+/* Synchronise the plan's last stream; SM_E_OVERFLOW if a launch overflowed. */
+int sm_pe_plan_sync(SmPePlan* plan);
+int sm_pe_plan_destroy(SmPePlan* plan);
+/* Algorithmic sub-ray count per element of the plan (voxel_sub^3 * sub_x*sub_y*sub_z). */
+long long sm_pe_plan_subrays(const SmPePlan* plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* SYSMAT_H_ */

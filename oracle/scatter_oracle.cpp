@@ -1,0 +1,413 @@
+// CPU oracle for the primary-Compton scatter system matrices.  TEST
+// INFRASTRUCTURE ONLY: the shipped library never links or calls this file.
+//
+// Restates, in the reference's operation order:
+//   geometryRelationShip_Crystal2Crystal    scatter.cu:1440-1540 (bitmap builder)
+//   geometryRelationShip_Collimator2Crystal scatter.cu:1939-2009 (bitmap builder)
+//   crystalScatterSysMatCuda                scatter.cu:1077-1437 (= ScatterGen_Crystal/scatter.cu:1061-1421)
+//   collimatorScatterSysMatCuda             scatter.cu:1544-1936 (virtual_r 0.1, projected disc)
+//       and its ScatterGen_Collimator variant (ScatterGen_Collimator/scatter.cu:1056-1436,
+//       virtual_r 0.2, solid angle (r^2 + dy r) / (4 L^2), :1212-1215)
+// One element (detector r, voxel v) sums the scatter sites (crystals k != r,
+// or layer-0 holes) in ascending order; the reference's atomicAdd order is
+// nondeterministic, so the FP32 sum order is a free choice.  The reference's
+// O(N) bitmap scan per (r, v, site) is done here once per (r, site) into an
+// attenuator list: the same set in the same order, independent of v.
+#include <omp.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "physics.hpp"
+
+namespace oracle {
+
+struct ScOpts {
+    int det_begin, det_end;  // detector rows
+    int threads;             // <= 0: all
+    int variant;             // collimator: 0 combined (ScatterGen_RayTracing_CircularHole), 1 ScatterGen_Collimator
+};
+
+static inline int nfloor(float v, float eps) { return (int)std::floor(v + eps); }
+
+// ---- bitmap builders (FP32, as the reference kernels) ----------------------
+static void build_crystal_bitmap(const float* det, uint32_t* words, long long n_words) {
+    const int n = (int)det[0];
+    std::memset(words, 0, sizeof(uint32_t) * (size_t)n_words);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int i = 0; i < n; ++i) {
+        const float* di = det + 12 * i;
+        float Ri = std::sqrt(di[4] * di[4] + di[6] * di[6] + di[5] * di[5]) / 2.0f;
+        for (int j = i + 1; j < n; ++j) {
+            const float* dj = det + 12 * j;
+            float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
+            float Lij = distance<float>(di[1], di[2], di[3], dj[1], dj[2], dj[3]);
+            float cij = Ri / Lij, cji = Rj / Lij;
+            float uijx = (di[1] - dj[1]) / Lij, uijy = (di[2] - dj[2]) / Lij, uijz = (di[3] - dj[3]) / Lij;
+            float ujix = (dj[1] - di[1]) / Lij, ujiy = (dj[2] - di[2]) / Lij, ujiz = (dj[3] - di[3]) / Lij;
+            long long pair = (long long)i * n - ((long long)i * (i + 1)) / 2 + j;
+            for (int k = 0; k < n; ++k) {
+                if (k == i || k == j) continue;
+                const float* dk = det + 12 * k;
+                float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
+                float Lik = distance<float>(dk[1], dk[2], dk[3], di[1], di[2], di[3]);
+                float cki = Rk / Lik;
+                float Ljk = distance<float>(dk[1], dk[2], dk[3], dj[1], dj[2], dj[3]);
+                float ckj = Rk / Ljk;
+                float ukix = (dk[1] - di[1]) / Lik, ukiy = (dk[2] - di[2]) / Lik, ukiz = (dk[3] - di[3]) / Lik;
+                float ukjx = (dk[1] - dj[1]) / Ljk, ukjy = (dk[2] - dj[2]) / Ljk, ukjz = (dk[3] - dj[3]) / Ljk;
+                bool cross = distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji;
+                if (!cross) cross = distance<float>(ukjx, ukjy, ukjz, uijx, uijy, uijz) <= ckj + cij;
+                if (cross) {
+                    long long bit = pair * n + k;
+#pragma omp atomic
+                    words[bit >> 5] |= (1u << (bit & 31));
+                }
+            }
+        }
+    }
+}
+
+static void build_collimator_bitmap(const float* col, const float* det, uint32_t* words, long long n_words) {
+    const int n = (int)det[0];
+    const int nh = (int)col[10];
+    std::memset(words, 0, sizeof(uint32_t) * (size_t)n_words);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int h = 0; h < nh; ++h) {
+        const float* hp = col + 100 + 9 * h;
+        float xh = hp[0], yh = (hp[1] + hp[2]) / 2.0f, zh = hp[3];
+        for (int j = 0; j < n; ++j) {
+            const float* dj = det + 12 * j;
+            float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
+            float Lij = distance<float>(xh, yh, zh, dj[1], dj[2], dj[3]);
+            float cji = Rj / Lij;
+            float ujix = (dj[1] - xh) / Lij, ujiy = (dj[2] - yh) / Lij, ujiz = (dj[3] - zh) / Lij;
+            for (int k = 0; k < n; ++k) {
+                const float* dk = det + 12 * k;
+                float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
+                float Lik = distance<float>(dk[1], dk[2], dk[3], xh, yh, zh);
+                float cki = Rk / Lik;
+                float ukix = (dk[1] - xh) / Lik, ukiy = (dk[2] - yh) / Lik, ukiz = (dk[3] - zh) / Lik;
+                if (distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji) {
+                    long long bit = collimator_bit(h, j, k, n);
+#pragma omp atomic
+                    words[bit >> 5] |= (1u << (bit & 31));
+                }
+            }
+        }
+    }
+}
+
+// ---- shared per-site evaluation (scatter.cu:1232-1430 / 1728-1929) ----------
+// Detector r's record and the physics set-up of its row.
+template <class R>
+struct Row {
+    R xD, yD, zD, w, h, t, rot, mu_t, mu_pe, mu_c, res;
+    R lo, hi, e0, kn_total;
+    R cr, sr, crn, srn;  // cos/sin(rot), cos/sin(-rot)
+};
+
+template <class R>
+static Row<R> make_row(const float* det, const float* img, const float* phy, int r) {
+    Row<R> w;
+    const float* d = det + 12 * r;
+    const R fov2c = R(img[11]);
+    w.xD = R(d[1]);
+    w.yD = R(d[2]) + fov2c;
+    w.zD = R(d[3]);
+    w.w = R(d[4]);
+    w.h = R(d[6]);
+    w.t = R(d[5]);
+    w.rot = R(d[11]);
+    w.mu_t = R(d[7]);
+    w.mu_pe = R(d[8]);
+    w.mu_c = R(d[9]);
+    w.res = R(d[10]);
+    w.e0 = R(phy[7]);
+    if (nfloor(phy[4], 0.5f) > 0) {
+        w.lo = R(phy[5]);
+        w.hi = R(phy[6]);
+    } else {
+        w.lo = (R(1.0f) - w.res / R(2.0f)) * w.e0;
+        w.hi = (R(1.0f) + w.res / R(2.0f)) * w.e0;
+    }
+    // normalisation integral over [0, pi] with pi rounded to the argument type
+    w.kn_total = kn_riemann(w.e0, R(0.0f), R((float)M_PI), R(0.01f));
+    w.cr = rot_cos(w.rot);
+    w.sr = rot_sin(w.rot);
+    w.crn = rot_cos(-w.rot);
+    w.srn = rot_sin(-w.rot);
+    return w;
+}
+
+// Contribution of one scatter site S to element (r, image point I); `atten`
+// lists the attenuating crystals of (site, r); `weight` is the site's source
+// term (prob_Compton_*).  Returns the product in the reference's mixed precision.
+template <class R>
+static R site_term(const Row<R>& D, const float* det, const float* img, R xI, R yI, R zI, R xS, R yS, R zS,
+                   R weight, const std::vector<int>& atten) {
+    using W = typename Wide<R>::type;
+    const R fov2c = R(img[11]);
+    // single sub-cell (divide 1x1x1): the sub-point is the crystal centre
+    const R xDs = -D.w / R(2.0f) + (R(0.0f) + R(0.5f)) / R(1.0f) * D.w;
+    const R zDs = -D.h / R(2.0f) + (R(0.0f) + R(0.5f)) / R(1.0f) * D.h;
+    const R yDs = -D.t / R(2.0f) + (R(0.0f) + R(0.5f)) / R(1.0f) * D.t;
+    const R xQ = D.xD + (xDs * D.cr - zDs * D.sr);
+    const R zQ = D.zD + (xDs * D.sr + zDs * D.cr);
+    const R yQ = D.yD + yDs;
+
+    const R theta = cone_angle(xI, yI, zI, xS, yS, zS, xQ, yQ, zQ);
+    const R es = scatter_energy(theta, D.e0);
+    const R res_s = D.res * r_sqrt(es / D.e0);
+    if (((es * (R(1.0f) + R(2.0f) * res_s / R(2.35482f))) <= D.lo) ||
+        (es * (R(1.0f) - R(2.0f) * res_s / R(2.35482f)) >= D.hi))
+        return R(0.0f);
+    const R p_e = window_probability(es, res_s, D.lo, D.hi);
+    const R L = distance(xS, yS, zS, xQ, yQ, zQ);
+    const R Rdet = r_sqrt(D.w * D.w / R(1.0f) / R(1.0f) + D.h * D.h / R(1.0f) / R(1.0f) +
+                          D.t * D.t / R(1.0f) / R(1.0f)) / R(2.0f);
+    R phi;
+    if (L * r_sin(theta) * R(2.0f) <= Rdet)
+        phi = R(W(R(2.0f)) * W(M_PI));
+    else
+        phi = R(4.0f) * r_asin(rmin(Rdet / R(2.0f) / L / r_sin(theta), R(1.0f)));
+    // theta range over the sub-cell corners, in the detector frame
+    const R xSs = (xS - D.xD) * D.crn - (zS - D.zD) * D.srn;
+    const R ySs = yS - D.yD;
+    const R zSs = (xS - D.xD) * D.srn + (zS - D.zD) * D.crn;
+    const R xIs = (xI - D.xD) * D.crn - (zI - D.zD) * D.srn;
+    const R yIs = yI - D.yD;
+    const R zIs = (xI - D.xD) * D.srn + (zI - D.zD) * D.crn;
+    const R ux1 = (R(0.0f) / R(1.0f) - R(0.5f)) * D.w, ux2 = ((R(0.0f) + R(1.0f)) / R(1.0f) - R(0.5f)) * D.w;
+    const R uy1 = (R(0.0f) / R(1.0f) - R(0.5f)) * D.t, uy2 = ((R(0.0f) + R(1.0f)) / R(1.0f) - R(0.5f)) * D.t;
+    const R uz1 = (R(0.0f) / R(1.0f) - R(0.5f)) * D.h, uz2 = ((R(0.0f) + R(1.0f)) / R(1.0f) - R(0.5f)) * D.h;
+    const R ext = R(1000.0f);
+    const R dIS = distance(xSs, ySs, zSs, xIs, yIs, zIs);
+    R xt = xSs + ext * (xSs - xIs) / dIS;
+    R yt = ySs + ext * (ySs - yIs) / dIS;
+    R zt = zSs + ext * (zSs - zIs) / dIS;
+    const R through = chord_box(xIs, yIs, zIs, xt, yt, zt, ux1, uy1, uz1, ux2, uy2, uz2);
+    R th[8];
+    th[0] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux1, uy1, uz1);
+    th[1] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux2, uy1, uz1);
+    th[2] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux1, uy2, uz1);
+    th[3] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux1, uy1, uz2);
+    th[4] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux2, uy2, uz1);
+    th[5] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux2, uy1, uz2);
+    th[6] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux1, uy2, uz2);
+    th[7] = cone_angle(xIs, yIs, zIs, xSs, ySs, zSs, ux2, uy2, uz2);
+    R tmin = th[0], tmax = th[0];
+    for (int i = 1; i < 8; ++i) {
+        if (th[i] > tmax) tmax = th[i];
+        if (th[i] < tmin) tmin = th[i];
+    }
+    if (through > R(0.001f)) tmin = R(0.0f);
+    const R ratio = kn_riemann(D.e0, tmin, tmax, R(0.01f)) / D.kn_total;
+    // attenuation site -> detector through the listed crystals, mu scaled to E'
+    R att = R(0.0f);
+    for (int k : atten) {
+        const float* dk = det + 12 * k;
+        R xk = R(dk[1]), yk = R(dk[2]) + fov2c, zk = R(dk[3]);
+        R wk = R(dk[4]), hk = R(dk[6]), tk = R(dk[5]), rk = R(dk[11]);
+        R pe = R(dk[8]), cc = R(dk[9]);
+        pe = pe * D.e0 * D.e0 * D.e0 / es / es / es;
+        cc = cc * D.e0 / es;
+        R tot = pe + cc;
+        R ck = rot_cos(-rk), sk = rot_sin(-rk);
+        R xSk = (xS - xk) * ck - (zS - zk) * sk, ySk = yS - yk, zSk = (xS - xk) * sk + (zS - zk) * ck;
+        R xQk = (xQ - xk) * ck - (zQ - zk) * sk, yQk = yQ - yk, zQk = (xQ - xk) * sk + (zQ - zk) * ck;
+        R c = chord_box(xSk, ySk, zSk, xQk, yQk, zQk, R(-0.5f) * wk, R(-0.5f) * tk, R(-0.5f) * hk, R(0.5f) * wk,
+                        R(0.5f) * tk, R(0.5f) * hk);
+        att = att + c * tot;
+    }
+    // inside the detecting crystal, before the (single) sub-cell
+    const R a1 = chord_box_open(xSs, ySs, zSs, xDs, yDs, zDs, -D.w / R(2.0f), -D.t / R(2.0f), -D.h / R(2.0f),
+                                D.w / R(2.0f), D.t / R(2.0f), D.h / R(2.0f));
+    const R a2 = chord_box_open(xSs, ySs, zSs, xDs, yDs, zDs, ux1, uy1, uz1, ux2, uy2, uz2);
+    const R pe_abs = D.mu_pe * D.e0 * D.e0 * D.e0 / es / es / es;
+    const R c_abs = D.mu_c * D.e0 / es;
+    const R tot_abs = pe_abs + c_abs;
+    att = att + (a1 - a2) * tot_abs;
+    xt = xDs + ext * (xDs - xSs) / L;
+    yt = yDs + ext * (yDs - ySs) / L;
+    zt = zDs + ext * (zDs - zSs) / L;
+    const R l_abs = chord_box(xSs, ySs, zSs, xt, yt, zt, ux1, uy1, uz1, ux2, uy2, uz2);
+    // product in the reference's evaluation order (M_PI promotes from the 3rd factor on)
+    W v = W(weight * phi / R(2.0f)) / W(M_PI);
+    v = v * W(ratio);
+    v = v * W(p_e);
+    v = v * W(r_exp(-att));
+    v = v * W(R(1.0f) - r_exp(-l_abs * tot_abs));
+    v = v * W(pe_abs);
+    v = v / W(tot_abs);
+    return R(v);
+}
+
+template <class R>
+static void voxel_point(const float* img, int vox, bool eps_counts, R& x, R& y, R& z) {
+    const float e = eps_counts ? 0.001f : 0.0f;
+    const int X = nfloor(img[0], e), Y = nfloor(img[1], e), Z = nfloor(img[2], e);
+    const int iz = vox / (Y * X);
+    const int rem = vox % (Y * X);
+    const int iy = rem / X, ix = rem % X;
+    R xi = (R((float)ix) - R((float)X) / R(2.0f) + R(0.5f)) * R(img[3]);
+    R yi = (R((float)iy) - R((float)Y) / R(2.0f) + R(0.5f)) * R(img[4]);
+    R zi = (R((float)iz) - R((float)Z) / R(2.0f) + R(0.5f)) * R(img[5]);
+    xi = xi + R(img[8]);
+    yi = yi + R(img[9]);
+    zi = zi + R(img[10]);
+    const R a = R(img[20]) * R(img[7]);
+    const R ca = rot_cos(a), sa = rot_sin(a);
+    x = xi * ca - yi * sa;
+    y = xi * sa + yi * ca;
+    z = zi;
+}
+
+// crystalScatterSysMatCuda over rows [det_begin, det_end) x all voxels.
+template <class R>
+static void crystal_run(const float* det, const float* img, const float* phy, const float* pe, const uint32_t* bits,
+                        float* dst, const ScOpts& o) {
+    const int n = nfloor(det[0], 0.0001f);
+    const int X = nfloor(img[0], 0.0f), Y = nfloor(img[1], 0.0f), Z = nfloor(img[2], 0.0f);
+    const long V = (long)X * Y * Z;
+    const int rows = o.det_end - o.det_begin;
+    const int nth = o.threads > 0 ? o.threads : omp_get_max_threads();
+    const bool use = nfloor(phy[0], 0.5f) == 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+    for (int rr = 0; rr < rows; ++rr) {
+        const int r = o.det_begin + rr;
+        Row<R> D = make_row<R>(det, img, phy, r);
+        std::vector<std::vector<int>> lists(n);
+        for (int k = 0; k < n; ++k) {
+            if (k == r) continue;
+            for (int a = 0; a < n; ++a)
+                if (a != r && a != k && test_bit(bits, crystal_pair_bit(k, r, a, n))) lists[k].push_back(a);
+        }
+        const R ratio_c = D.mu_c / D.mu_pe;  // mu_c / mu_pe of the detecting crystal (scatter.cu:1218-1219)
+        for (long v = 0; v < V; ++v) {
+            R acc = R(0.0f);
+            if (use && D.mu_t > R(0.01f)) {
+                R xI, yI, zI;
+                voxel_point<R>(img, (int)v, false, xI, yI, zI);
+                for (int k = 0; k < n; ++k) {
+                    if (k == r) continue;
+                    const float* dk = det + 12 * k;
+                    const R w = R(pe[(long)k * V + v]) * D.mu_c / D.mu_pe;
+                    (void)ratio_c;
+                    acc += site_term<R>(D, det, img, xI, yI, zI, R(dk[1]), R(dk[2]) + R(img[11]), R(dk[3]), w,
+                                        lists[k]);
+                }
+            }
+            dst[(long)rr * V + v] = (float)to_double(acc);
+        }
+    }
+}
+
+// collimatorScatterSysMatCuda (variant 0) / ScatterGen_Collimator (variant 1).
+template <class R>
+static void collimator_run(const float* col, const float* det, const float* img, const float* phy,
+                           const uint32_t* bits, float* dst, const ScOpts& o) {
+    using W = typename Wide<R>::type;
+    const int n = nfloor(det[0], 0.0001f);
+    const int nh = nfloor(col[10], 0.000001f);
+    const bool eps_counts = (o.variant == 0);
+    const float e = eps_counts ? 0.001f : 0.0f;
+    const int X = nfloor(img[0], e), Y = nfloor(img[1], e), Z = nfloor(img[2], e);
+    const long V = (long)X * Y * Z;
+    const int rows = o.det_end - o.det_begin;
+    const int nth = o.threads > 0 ? o.threads : omp_get_max_threads();
+    const bool use = nfloor(phy[0], 0.5f) == 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+    for (int rr = 0; rr < rows; ++rr) {
+        const int r = o.det_begin + rr;
+        Row<R> D = make_row<R>(det, img, phy, r);
+        std::vector<std::vector<int>> lists(nh);
+        for (int h = 0; h < nh; ++h)
+            for (int a = 0; a < n; ++a)
+                if (a != r && test_bit(bits, collimator_bit(h, r, a, n))) lists[h].push_back(a);
+        for (long v = 0; v < V; ++v) {
+            R acc = R(0.0f);
+            if (use && D.mu_t > R(0.01f)) {
+                R xI, yI, zI;
+                voxel_point<R>(img, (int)v, eps_counts, xI, yI, zI);
+                for (int h = 0; h < nh; ++h) {
+                    const float* hp = col + 100 + 9 * h;
+                    const R xS = R(hp[0]);
+                    const R yS = (R(hp[1]) + R(hp[2])) / R(2.0f) + R(img[11]);
+                    const R zS = R(hp[3]);
+                    const R L = distance(xI, yI, zI, xS, yS, zS);
+                    R solid;
+                    if (o.variant == 0) {
+                        const R vr = R(0.1f);
+                        W num = W(vr * vr) * W(M_PI) + W((R(hp[2]) - R(hp[1])) * vr);
+                        W den = W(R(4.0f)) * W(M_PI) * W(L) * W(L);
+                        solid = R(num / den * W(r_fabs(yS - yI)) / W(L));
+                    } else {
+                        const R vr = R(0.2f);
+                        solid = (vr * vr + (R(hp[2]) - R(hp[1])) * vr) / (R(4.0f) * L * L);
+                    }
+                    const R w = solid / R(col[15]) * R(col[17]);
+                    acc += site_term<R>(D, det, img, xI, yI, zI, xS, yS, zS, w, lists[h]);
+                }
+            }
+            dst[(long)rr * V + v] = (float)to_double(acc);
+        }
+    }
+}
+
+}  // namespace oracle
+
+using oracle::ScOpts;
+
+extern "C" {
+
+long long or_crystal_bitmap_words(int n) {
+    long long pairs = (long long)n * (n + 1) / 2;
+    return (pairs * n + 31) / 32;
+}
+
+long long or_collimator_bitmap_words(int n_holes, int n) { return ((long long)n * n * n_holes + 31) / 32; }
+
+int or_bitmap_crystal(const float* det, uint32_t* words, long long n_words) {
+    oracle::build_crystal_bitmap(det, words, n_words);
+    return 0;
+}
+
+int or_bitmap_collimator(const float* col, const float* det, uint32_t* words, long long n_words) {
+    oracle::build_collimator_bitmap(col, det, words, n_words);
+    return 0;
+}
+
+int or_scatter_crystal(const float* det, const float* img, const float* phy, const float* pe, const uint32_t* bits,
+                       float* dst, const ScOpts* o, int f64) {
+    if (f64)
+        oracle::crystal_run<double>(det, img, phy, pe, bits, dst, *o);
+    else
+        oracle::crystal_run<float>(det, img, phy, pe, bits, dst, *o);
+    return 0;
+}
+
+int or_scatter_collimator(const float* col, const float* det, const float* img, const float* phy,
+                          const uint32_t* bits, float* dst, const ScOpts* o, int f64) {
+    if (f64)
+        oracle::collimator_run<double>(col, det, img, phy, bits, dst, *o);
+    else
+        oracle::collimator_run<float>(col, det, img, phy, bits, dst, *o);
+    return 0;
+}
+
+float or_kn_riemann_f32(float e0, float lo, float hi) { return oracle::kn_riemann<float>(e0, lo, hi, 0.01f); }
+double or_kn_riemann_f64(double e0, double lo, double hi) {
+    return oracle::kn_riemann<double>(e0, lo, hi, (double)0.01f);
+}
+float or_scatter_energy_f32(float theta, float e0) { return oracle::scatter_energy<float>(theta, e0); }
+float or_window_probability_f32(float e, float res, float lo, float hi) {
+    return oracle::window_probability<float>(e, res, lo, hi);
+}
+float or_cone_angle_f32(const float* a) {
+    return oracle::cone_angle<float>(a[0], a[1], a[2], a[3], a[4], a[5], a[6], a[7], a[8]);
+}
+
+}  // extern "C"

@@ -1,0 +1,61 @@
+"""CPU-side checks of the C ABI library and the host logic (no compute calls)."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+from sysmat import _lib, params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    text = open(os.path.join(ROOT, "include", "sysmat.h")).read()
+    return sorted(set(re.findall(r"\b(sm_\w+)\s*\(", text)))
+
+
+def test_library_exports_every_declared_symbol():
+    assert os.path.exists(_lib.LIB_PATH), "libsysmat_hip.so not built"
+    L = _lib.load()
+    names = header_functions()
+    assert names, "no sm_* declarations found"
+    for n in names:
+        assert hasattr(L, n), f"{n} declared in include/sysmat.h but not exported"
+        assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
+    # and the C++-linkage reference drop-in (PESysMatGen.h:4-9)
+    assert hasattr(L, "_Z11PESysMatGenPfS_S_S_i")
+
+
+def test_version_and_errors_without_gpu():
+    L = _lib.load()
+    assert L.sm_version() >= 100
+    assert L.sm_device_count() >= 0
+
+
+def test_params_roundtrip(tmp_path):
+    g = params.config2(seed=3)
+    g.write(str(tmp_path))
+    h = params.Geometry.read(str(tmp_path))
+    for a, b in ((g.col, h.col), (g.det, h.det), (g.img, h.img), (g.phy, h.phy)):
+        assert np.array_equal(a, b)
+    assert h.n_det == 16384 and h.n_layers == 2 and h.n_holes == 128
+    assert h.shape == (64, 64, 64)
+
+
+def test_short_param_files_read_zero_filled(tmp_path):
+    g = params.config1()
+    g.write(str(tmp_path))
+    np.asarray(g.img[:12], "<f4").tofile(str(tmp_path / "Params_Image.dat"))
+    h = params.Geometry.read(str(tmp_path))
+    assert h.img.size == 100 and np.all(h.img[12:] == 0)
+
+
+def test_config_shapes():
+    c1 = params.config1()
+    assert c1.n_det == 1024 and c1.shape == (16, 16, 16) and c1.n_holes == 25
+    c3 = params.config3()
+    assert c3.n_det == 2048 and c3.n_holes == 36
+    vs = params.config4_variants(seed=0)
+    assert len(vs) == 6 and all(v.n_det == 6912 for v in vs)
+    assert 1 <= min(v.n_layers for v in vs) and max(v.n_layers for v in vs) <= 3
