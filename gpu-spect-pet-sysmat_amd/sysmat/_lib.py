@@ -66,6 +66,16 @@ def load(path: str = LIB_PATH):
         return _lib
     if not os.path.exists(path):
         raise SysmatError(SM_E_DEVICE, f"{path} is not built; run __graft_entry__.build() (make -C {PKG_DIR})")
+    # One HIP runtime per process: PyTorch-ROCm ships its own libamdhip64.so.7
+    # (same soname).  Importing torch first makes the dynamic loader bind this
+    # library to torch's runtime, so device tensors, streams and events are
+    # shared; loading ours first would leave torch with a second runtime that
+    # sees no GPU.
+    if os.environ.get("SYSMAT_NO_TORCH", "0") != "1":
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
     L = ctypes.CDLL(path)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(L, name)
