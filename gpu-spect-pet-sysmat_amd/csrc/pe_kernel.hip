@@ -11,16 +11,15 @@
 //   * one workgroup = 4 wave64s = one detector x one voxel tile;
 //   * the workgroup culls all crystals / holes once against the tile's
 //     bounding sphere (a proven superset of every element's cull list) into LDS;
-//   * one wave = one element at a time, its 64 lanes = 64 sub-rays; the exact
-//     per-element cull predicate is evaluated lane-parallel over the tile
-//     candidates and kept as a wave-uniform bitmask in LDS, so the attenuator
-//     loop is uniform (no divergence) and reads crystal records with scalar loads;
-//   * a per-element "bundle" filter bounds every sub-ray of the element
-//     laterally in each y-slab (exact by per-variable monotonicity, evaluated
-//     at the 32 vertex rays), dropping attenuators no sub-ray can touch --
-//     they would contribute a zero chord, so results are unchanged;
+//   * one wave = one element at a time, its 64 lanes = 64 sub-rays.  Per
+//     element the wave filters the tile candidates lane-parallel -- first the
+//     "bundle" test (every sub-ray of the element bounded laterally in each
+//     y-slab, exact by per-variable monotonicity at the 32 vertex rays: a
+//     crystal outside it has a zero chord for every sub-ray), then the exact
+//     reference cull predicate -- and compacts the survivors' boxes into LDS,
+//     so the attenuator loop is uniform across the wave;
 //   * FP32 tracing; FP64 for the element accumulator and the collimator path
-//     (hole chords and plate length), per the precision plan of the design.
+//     (hole chords and plate length from FP64 end points).
 #include <hip/hip_runtime.h>
 
 #include "pe_kernel.hpp"
@@ -30,6 +29,8 @@ namespace sysmat {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
+constexpr int kListCap = 192;   // per-wave compacted attenuator boxes (beyond: bitmask path)
+constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
 
 __device__ inline float wave_min32(float v) {
 #pragma unroll
@@ -52,6 +53,16 @@ __device__ inline T uniform(T v) {
     return __builtin_amdgcn_readfirstlane(v);
 }
 
+// q = s / d, r = s % d for 0 <= s < 2^24 via a float reciprocal and one correction.
+__device__ inline int divmod(int s, int d, float rd, int& r) {
+    int q = (int)((float)s * rd);
+    int m = s - q * d;
+    if (m < 0) { q -= 1; m += d; }
+    else if (m >= d) { q += 1; m -= d; }
+    r = m;
+    return q;
+}
+
 // Voxel centre (before image rotation) of voxel (ix, iy, iz) -- PESysMatGen.cu:438-444.
 __device__ inline void voxel_centre(const PeParams& p, int ix, int iy, int iz, float& x, float& y, float& z) {
     x = ((float)ix - (float)p.X / 2.0f + 0.5f) * p.wx + p.sx;
@@ -64,21 +75,53 @@ __device__ inline void rotate_image(const PeParams& p, float x, float y, float& 
     yr = x * p.rs + y * p.rc;
 }
 
-__global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry,
+// Compacted attenuator (32 B in LDS): world box and mu for crystals with no
+// rotation; rotated crystals carry their index and are traced in their frame.
+struct Att {
+    float x1, x2, y1, y2, z1, z2, mu;
+    int k;  // crystal index; negative (-(k+1)) when the crystal is rotated
+};
+
+// Chord of the sub-ray through crystal k (reference order: PESysMatGen.cu:687-725).
+__device__ inline float crystal_chord(const WRay& R, const Crystal* __restrict__ cry, int k, float x1, float x2,
+                                      float y1, float y2, float z1, float z2, bool rotated) {
+    if (!rotated && R.pat == 0) return chord_world(R, x1, x2, y1, y2, z1, z2);
+    const Crystal K = cry[k];
+    if (!rotated) {  // along-axis branches, compared in the crystal frame
+        const float px = R.px - K.cx, py = R.py - K.cy, pz = R.pz - K.cz;
+        return chord_special(px, py, pz, px + R.dx, py + R.dy, pz + R.dz, R.ix, R.iy, R.iz, R.t, R.pat, -K.hx, -K.hy,
+                             -K.hz, K.hx, K.hy, K.hz);
+    }
+    const float qx = R.px + R.dx, qy = R.py + R.dy, qz = R.pz + R.dz;
+    const float xPk = (R.px - K.cx) * K.cr - (R.pz - K.cz) * K.sr, yPk = R.py - K.cy;
+    const float zPk = (R.px - K.cx) * K.sr + (R.pz - K.cz) * K.cr;
+    const float xQk = (qx - K.cx) * K.cr - (qz - K.cz) * K.sr, yQk = qy - K.cy;
+    const float zQk = (qx - K.cx) * K.sr + (qz - K.cz) * K.cr;
+    const WRay Rk = make_wray(xPk, yPk, zPk, xQk, yQk, zQk);
+    if (Rk.pat != 0)
+        return chord_special(xPk, yPk, zPk, xQk, yQk, zQk, Rk.ix, Rk.iy, Rk.iz, Rk.t, Rk.pat, -K.hx, -K.hy, -K.hz,
+                             K.hx, K.hy, K.hz);
+    return chord_world(Rk, -K.hx, K.hx, -K.hy, K.hy, -K.hz, K.hz);
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry,
                                                            const Hole* __restrict__ hol,
                                                            const Plate* __restrict__ pla,
                                                            const Slab* __restrict__ slb, float* __restrict__ dst,
                                                            int* __restrict__ status) {
-    __shared__ unsigned s_cand[kCapCrystals];
-    __shared__ unsigned s_hcand[kCapHoles];
+    __shared__ unsigned short s_cand[kCapCrystals];
+    __shared__ unsigned short s_hcand[kCapHoles];
     __shared__ unsigned s_mask[kWaves][kCapCrystals / 32];
     __shared__ unsigned s_hmask[kWaves][kCapHoles / 32];
+    __shared__ Att s_att[kWaves][kListCap];
+    __shared__ int s_hol[kWaves][kHoleCap];
     __shared__ float4 s_box[kWaves][kMaxSlabs];
     __shared__ int s_wcount[kWaves];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
     const int wave = tid >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
     const long long wg = blockIdx.x;
     const int drow = (int)(wg % p.det_count);
@@ -89,14 +132,15 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
     const int tiz = (int)(tile / ((long long)p.ntx * p.nty));
     const int vx0 = tix * p.tx, vy0 = tiy * p.ty, vz0 = tiz * p.tz;
     const int vx1 = min(vx0 + p.tx, p.X), vy1 = min(vy0 + p.ty, p.Y), vz1 = min(vz0 + p.tz, p.Zslab);
-    const int nvox = (vx1 - vx0) * (vy1 - vy0) * (vz1 - vz0);
+    const int nvx = vx1 - vx0, nvy = vy1 - vy0;
+    const int nvox = nvx * nvy * (vz1 - vz0);
 
     const Crystal D = cry[d];
     float* row = dst + (long long)drow * p.row_stride;
 
-    if (!(D.mu_t > 0.01f)) {  // no culling, no sub-rays: the element is 0
+    if (!(D.mu_t > 0.01f)) {  // no culling, no sub-rays: the element is 0 (PESysMatGen.cu:504, 598)
         for (int e = tid; e < nvox; e += kThreads) {
-            int lx = e % (vx1 - vx0), ly = (e / (vx1 - vx0)) % (vy1 - vy0), lz = e / ((vx1 - vx0) * (vy1 - vy0));
+            const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
             row[(long long)(vz0 + lz) * p.X * p.Y + (vy0 + ly) * p.X + (vx0 + lx)] = 0.0f;
         }
         return;
@@ -106,33 +150,38 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
     float ax, ay, az, bx, by, bz;
     voxel_centre(p, vx0, vy0, p.z0 + vz0, ax, ay, az);
     voxel_centre(p, vx1 - 1, vy1 - 1, p.z0 + vz1 - 1, bx, by, bz);
-    float tcx0 = 0.5f * (ax + bx), tcy0 = 0.5f * (ay + by), tcz = 0.5f * (az + bz);
     float tcx, tcy;
-    rotate_image(p, tcx0, tcy0, tcx, tcy);
-    const float rho = 0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
+    rotate_image(p, 0.5f * (ax + bx), 0.5f * (ay + by), tcx, tcy);
+    const float tcz = 0.5f * (az + bz);
+    const float rho =
+        0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
 
     // ---- workgroup cull: superset of every element's list ----------------
-    // Ordered compaction (ballot + wave prefix), so candidate lists are sorted
-    // by index and the kernel is deterministic run to run.
+    // For any voxel centre I within rho of the tile centre I0, the reference
+    // predicate |u_k - u_d| <= r_k/D_k + r_d/D_d implies
+    // |u_k(I0) - u_d(I0)| <= r_k/(D_k0 - rho) + r_d/(D_d0 - rho) + 2 rho/D_k0 + 2 rho/D_d0
+    // (|a/|a| - b/|b|| <= 2|a - b|/|b|).  Ordered compaction keeps the lists
+    // sorted by index, so every element visits attenuators in index order.
     int nc = 0, nh = 0;
     {
         const float ddx = D.cx - tcx, ddy = D.cy - tcy, ddz = D.cz - tcz;
         const float Dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
         const float udx = ddx / Dd, udy = ddy / Dd, udz = ddz / Dd;
         const float crit_d = (Dd > rho) ? 0.5f * D.rbound / (Dd - rho) + 2.0f * rho / Dd : 4.0f;
-        const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
         for (int base = 0; base < p.n_det; base += kThreads) {
             const int k = base + tid;
             bool take = false;
             if (k < p.n_det && k != d) {
-                const Crystal K = cry[k];
-                float kx = K.cx - tcx, ky = K.cy - tcy, kz = K.cz - tcz;
-                float Dk = sqrtf(kx * kx + ky * ky + kz * kz);
+                const float4 c0 = reinterpret_cast<const float4*>(cry + k)[0];
+                const float rb = cry[k].rbound;
+                const float kx = c0.x - tcx, ky = c0.y - tcy, kz = c0.z - tcz;
+                const float Dk = sqrtf(kx * kx + ky * ky + kz * kz);
                 if (Dk <= rho) {
                     take = true;
                 } else {
-                    float bound = 0.5f * K.rbound / (Dk - rho) + 2.0f * rho / Dk + crit_d;
-                    float vx = kx / Dk - udx, vy = ky / Dk - udy, vz = kz / Dk - udz;
+                    const float iD = 1.0f / Dk;
+                    const float bound = 0.5f * rb / (Dk - rho) + 2.0f * rho * iD + crit_d;
+                    const float vx = kx * iD - udx, vy = ky * iD - udy, vz = kz * iD - udz;
                     take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
                 }
             }
@@ -146,7 +195,7 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
                 all += c;
             }
             const int slot = off + __popcll(bal & below);
-            if (take && slot < kCapCrystals) s_cand[slot] = (unsigned)k;
+            if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
             nc += all;
             __syncthreads();
         }
@@ -155,13 +204,14 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
             bool take = false;
             if (h < p.n_holes) {
                 const Hole H = hol[h];
-                float hx = H.x - tcx, hy = 0.5f * (H.y1 + H.y2) - tcy, hz = H.z - tcz;
-                float Dh = sqrtf(hx * hx + hy * hy + hz * hz);
+                const float hx = H.x - tcx, hy = 0.5f * (H.y1 + H.y2) - tcy, hz = H.z - tcz;
+                const float Dh = sqrtf(hx * hx + hy * hy + hz * hz);
                 if (Dh <= rho) {
                     take = true;
                 } else {
-                    float bound = H.rbound / (Dh - rho) + 2.0f * rho / Dh + crit_d;
-                    float vx = hx / Dh - udx, vy = hy / Dh - udy, vz = hz / Dh - udz;
+                    const float iD = 1.0f / Dh;
+                    const float bound = H.rbound / (Dh - rho) + 2.0f * rho * iD + crit_d;
+                    const float vx = hx * iD - udx, vy = hy * iD - udy, vz = hz * iD - udz;
                     take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
                 }
             }
@@ -175,7 +225,7 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
                 all += c;
             }
             const int slot = off + __popcll(bal & below);
-            if (take && slot < kCapHoles) s_hcand[slot] = (unsigned)h;
+            if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
             nh += all;
             __syncthreads();
         }
@@ -186,21 +236,27 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
     }
 
     const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
-    const int nsub_v = p.nv * p.nv * p.nv;
     const int S = p.S;
-    const float inv_nv3 = 1.0f / (float)nsub_v;
+    const float inv_nv3 = 1.0f / (float)(p.nv * p.nv * p.nv);
     // crystal-d sub-point geometry
     const float wD = 2.0f * D.hx, tD = 2.0f * D.hy, hD = 2.0f * D.hz;
-    const float cD = cosf(D.rot), sD = sinf(D.rot);  // rotation of the sub-points into the world
+    const float cD = cosf(D.rot), sD = sinf(D.rot);      // sub-points into the world
+    const float cDn = cosf(-D.rot), sDn = sinf(-D.rot);  // world into the crystal frame
     const double cD64 = (D.rot == 0.0f) ? 1.0 : cos((double)D.rot);
     const double sD64 = (D.rot == 0.0f) ? 0.0 : sin((double)D.rot);
     const float fcx = (float)p.cx, fcy = (float)p.cy, fcz = (float)p.cz;
+    const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
+    const float absorb_scale = D.mu_pe / D.mu_t;
     // world AABB of crystal d (bundle filter end box)
     const float qex = fabsf(cD) * D.hx + fabsf(sD) * D.hz, qez = fabsf(sD) * D.hx + fabsf(cD) * D.hz;
+    const float hvx = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
+    const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
+    const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
+    const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
 
     for (int e = wave; e < nvox; e += kWaves) {
-        const int lx = e % (vx1 - vx0), ly = (e / (vx1 - vx0)) % (vy1 - vy0), lz = e / ((vx1 - vx0) * (vy1 - vy0));
+        const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
         const int ix = vx0 + lx, iy = vy0 + ly, izl = vz0 + lz;
         float xc0, yc0, zc0;
         voxel_centre(p, ix, iy, p.z0 + izl, xc0, yc0, zc0);
@@ -214,10 +270,6 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
         const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
 
         // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (crystal d)
-        const float hvx = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
-        const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
-        const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
-        const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
         const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - D.hy, Q_yhi = D.cy + D.hy;
         const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
         if (bundle_ok) {
@@ -239,55 +291,87 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
             }
         }
         __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
 
-        // -- crystal mask: exact predicate && bundle overlap
+        // -- crystals: bundle overlap, then the exact predicate; survivors are
+        //    kept as a bitmask and, when they fit, as compacted boxes
+        int n_att = 0;
         for (int w = 0; w < ncw; w += 2) {
             const int j = w * 32 + lane;
             bool take = false;
+            Crystal K;
+            int k = 0;
             if (j < nc) {
-                const int k = (int)s_cand[j];
-                const Crystal K = cry[k];
-                float crit = K.rbound;
-                float dk = sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) + (zI - K.cz) * (zI - K.cz));
-                crit = crit / dk / 2.0f;
-                float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
-                take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <= crit + crit_self;
-                if (take && bundle_ok && K.slab >= 0.0f) {
+                k = (int)s_cand[j];
+                K = cry[k];
+                take = true;
+                if (bundle_ok && K.slab >= 0.0f) {
                     const float4 bb = s_box[wave][(int)K.slab];
                     const float ex = (K.axis != 0.0f) ? K.hx : sqrtf(K.hx * K.hx + K.hz * K.hz);
                     const float ez = (K.axis != 0.0f) ? K.hz : ex;
                     take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
                 }
+                if (take) {
+                    float crit = K.rbound;
+                    const float dk =
+                        sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) + (zI - K.cz) * (zI - K.cz));
+                    crit = crit / dk / 2.0f;
+                    const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
+                    take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                           crit + crit_self;
+                }
             }
             const unsigned long long bal = __ballot(take);
+            if (take) {
+                const int slot = n_att + __popcll(bal & below);
+                if (slot < kListCap) {
+                    Att A;
+                    A.x1 = K.cx - K.hx; A.x2 = K.cx + K.hx;
+                    A.y1 = K.cy - K.hy; A.y2 = K.cy + K.hy;
+                    A.z1 = K.cz - K.hz; A.z2 = K.cz + K.hz;
+                    A.mu = K.mu_t;
+                    A.k = (K.axis != 0.0f) ? k : -(k + 1);
+                    s_att[wave][slot] = A;
+                }
+            }
+            n_att += __popcll(bal);
             if (lane == 0) {
                 s_mask[wave][w] = (unsigned)bal;
                 if (w + 1 < (kCapCrystals / 32)) s_mask[wave][w + 1] = (unsigned)(bal >> 32);
             }
         }
-        // -- hole mask: exact predicate && bundle overlap; layer_holes bit m set
-        //    when some hole of layer m survives
+        // -- holes: bundle overlap, exact predicate; layer_holes bit m set when
+        //    some hole of layer m survives
         unsigned layer_holes = 0;
+        int n_hol = 0;
         for (int w = 0; w < nhw; w += 2) {
             const int j = w * 32 + lane;
             bool take = false;
+            int h = 0;
             if (j < nh) {
-                const int h = (int)s_hcand[j];
+                h = (int)s_hcand[j];
                 const Hole H = hol[h];
-                const float yh = (H.y1 + H.y2) / 2.0f;
-                float crit = H.rbound;
-                float dh = sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
-                crit = crit / dh;
-                float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
-                take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <= crit + crit_self;
-                if (take && bundle_ok && H.slab >= 0.0f) {
+                take = true;
+                if (bundle_ok && H.slab >= 0.0f) {
                     const float4 bb = s_box[wave][(int)H.slab];
                     take = !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+                }
+                if (take) {
+                    const float yh = (H.y1 + H.y2) / 2.0f;
+                    float crit = H.rbound;
+                    const float dh = sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
+                    crit = crit / dh;
+                    const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
+                    take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                           crit + crit_self;
                 }
                 if (take) layer_holes |= 1u << (int)H.layer;
             }
             const unsigned long long bal = __ballot(take);
+            if (take) {
+                const int slot = n_hol + __popcll(bal & below);
+                if (slot < kHoleCap) s_hol[wave][slot] = h;
+            }
+            n_hol += __popcll(bal);
             if (lane == 0) {
                 s_hmask[wave][w] = (unsigned)bal;
                 if (w + 1 < (kCapHoles / 32)) s_hmask[wave][w + 1] = (unsigned)(bal >> 32);
@@ -296,7 +380,9 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
 #pragma unroll
         for (int o = 32; o >= 1; o >>= 1) layer_holes |= (unsigned)__shfl_xor((int)layer_holes, o, 64);
         layer_holes = uniform(layer_holes);
-        // -- plate classes: 0 test per sub-ray, 1 every sub-ray passes, 2 every sub-ray misses
+        n_att = uniform(n_att);
+        n_hol = uniform(n_hol);
+        // -- plate classes: test per sub-ray, every sub-ray passes, every sub-ray misses
         unsigned plate_test = 0;  // bit m: per-sub-ray test needed
         bool element_blocked = false;
         for (int m = 0; m < p.n_layers; ++m) {
@@ -318,28 +404,28 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
 
         double acc = 0.0;
         if (!element_blocked) {
-            // image point in the crystal-d frame (PESysMatGen.cu:460-464), voxel centre
-            const float cDn = cosf(-D.rot), sDn = sinf(-D.rot);
             // FP64 voxel centre (before rotation) for the collimator path
             const double xc64 = ((double)ix - (double)p.X / 2.0 + 0.5) * (double)p.wx + (double)p.sx;
             const double yc64 = ((double)iy - (double)p.Y / 2.0 + 0.5) * (double)p.wy + (double)p.sy;
             const double zc64 = ((double)(p.z0 + izl) - (double)p.Zfull / 2.0 + 0.5) * (double)p.wz + (double)p.sz;
+            const bool att_fast = n_att <= kListCap;
+            const bool hol_fast = n_hol <= kHoleCap;
             for (int base = 0; base < S; base += 64) {
                 const int s = base + lane;
                 if (s >= S) break;
                 // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
-                int rr = s;
-                const int ny = rr % p.cy; rr /= p.cy;
-                const int nx = rr % p.cx; rr /= p.cx;
-                const int nz = rr % p.cz; rr /= p.cz;
-                const int iv = rr;
+                int ny, nx, nz;
+                int q = divmod(s, p.cy, rcy, ny);
+                q = divmod(q, p.cx, rcx, nx);
+                const int iv = divmod(q, p.cz, rcz, nz);
                 float xP = xI, yP = yI, zP = zI;
+                int va = 0, vb = 0, vc = 0;
                 if (p.nv > 1) {
-                    const int a = iv % p.nv, b = (iv / p.nv) % p.nv, c = iv / (p.nv * p.nv);
-                    const float fnv = (float)p.nv;
-                    const float ox = (((float)a + 0.5f) / fnv - 0.5f) * p.wx;
-                    const float oy = (((float)b + 0.5f) / fnv - 0.5f) * p.wy;
-                    const float oz = (((float)c + 0.5f) / fnv - 0.5f) * p.wz;
+                    const int t1 = divmod(iv, p.nv, rnv, va);
+                    vc = divmod(t1, p.nv, rnv, vb);
+                    const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
+                    const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
+                    const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
                     rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
                     zP = zc0 + oz;
                 }
@@ -347,116 +433,122 @@ __global__ void __launch_bounds__(kThreads) pe_tile_kernel(PeParams p, const Cry
                 const float yPs = yP - D.cy;
                 const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
                 // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
-                const float xDs = -wD / 2.0f + ((float)nx + 0.5f) / fcx * wD;
-                const float zDs = -hD / 2.0f + ((float)nz + 0.5f) / fcz * hD;
-                const float yDs = -tD / 2.0f + ((float)ny + 0.5f) / fcy * tD;
+                const float xDs = -wD / 2.0f + ((float)nx + 0.5f) * rcx * wD;
+                const float zDs = -hD / 2.0f + ((float)nz + 0.5f) * rcz * hD;
+                const float yDs = -tD / 2.0f + ((float)ny + 0.5f) * rcy * tD;
                 const float xQ = D.cx + (xDs * cD - zDs * sD);
                 const float zQ = D.cz + (xDs * sD + zDs * cD);
                 const float yQ = D.cy + yDs;
 
-                const Ray R = make_ray(xP, yP, zP, xQ, yQ, zQ);
-                const float dx = R.dx, dy = R.dy, dz = R.dz;
-                const float dsq = dy * dy + dx * dx + dz * dz;
+                const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
                 const float dist = R.t;
                 // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
-                const float fx = A_x * fabsf(dx * cD - dz * sD);
-                const float fz = A_z * fabsf(dx * sD + dz * cD);
-                const float fy = A_y * fabsf(dy);
-                const float solid = fmaxf(fmaxf(fx, fy), fz) * (p.inv4pi / (dsq * dist));
+                const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
+                const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
+                const float fy = A_y * fabsf(R.dy);
+                const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
 
                 bool blocked = false;
-                for (unsigned mm = plate_test; mm; mm &= mm - 1) {
+                for (unsigned mm = plate_test; mm; mm &= mm - 1) {  // plate miss => blocked (PESysMatGen.cu:645-659)
                     const int m = __builtin_ctz(mm);
                     const Plate PL = pla[m];
-                    if (chord_through(R, PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2) < 0.1f) blocked = true;
+                    const float c = (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
+                                                 : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t,
+                                                                 R.pat, PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
+                    if (c < 0.1f) blocked = true;
                 }
                 float att = 0.0f;
                 double att64 = 0.0;
-                // collimator layers: plate length minus hole chords
-                {
-                    bool r64_ready = false;
-                    Ray64 R64;
+                // collimator layers: plate length minus hole chords (PESysMatGen.cu:661-685)
+                if (layer_holes == 0) {
+                    const float icos = dist * frcp(R.dy);
                     for (int m = 0; m < p.n_layers; ++m) {
                         const Plate PL = pla[m];
-                        if (!((layer_holes >> m) & 1u)) {  // no hole of this layer can be hit
-                            const float in_plate = PL.thick / (dy / dist);
-                            if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
-                            continue;
-                        }
-                        if (!r64_ready) {
-                            // end points in FP64 with the same formulas
-                            double px = xc64, py = yc64, pz = zc64;
-                            if (p.nv > 1) {
-                                const int a = iv % p.nv, b = (iv / p.nv) % p.nv, c = iv / (p.nv * p.nv);
-                                const double fnv = (double)p.nv;
-                                px += (((double)a + 0.5) / fnv - 0.5) * (double)p.wx;
-                                py += (((double)b + 0.5) / fnv - 0.5) * (double)p.wy;
-                                pz += (((double)c + 0.5) / fnv - 0.5) * (double)p.wz;
-                            }
-                            const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
-                            const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
-                            const double xs = -w64 / 2.0 + ((double)nx + 0.5) / (double)p.cx * w64;
-                            const double zs = -h64 / 2.0 + ((double)nz + 0.5) / (double)p.cz * h64;
-                            const double ys = -t64 / 2.0 + ((double)ny + 0.5) / (double)p.cy * t64;
-                            const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
-                            const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
-                            const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
-                            R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
-                            r64_ready = true;
-                        }
+                        const float in_plate = PL.thick * icos;
+                        if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
+                    }
+                } else {
+                    // end points in FP64 with the same formulas
+                    double px = xc64, py = yc64, pz = zc64;
+                    if (p.nv > 1) {
+                        const double fnv = (double)p.nv;
+                        px += (((double)va + 0.5) / fnv - 0.5) * (double)p.wx;
+                        py += (((double)vb + 0.5) / fnv - 0.5) * (double)p.wy;
+                        pz += (((double)vc + 0.5) / fnv - 0.5) * (double)p.wz;
+                    }
+                    const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
+                    const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
+                    const double xs = -w64 / 2.0 + ((double)nx + 0.5) / (double)p.cx * w64;
+                    const double zs = -h64 / 2.0 + ((double)nz + 0.5) / (double)p.cz * h64;
+                    const double ys = -t64 / 2.0 + ((double)ny + 0.5) / (double)p.cy * t64;
+                    const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
+                    const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
+                    const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
+                    const Ray64 R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
+                    for (int m = 0; m < p.n_layers; ++m) {
+                        const Plate PL = pla[m];
                         double holes_len = 0.0;
-                        for (int w = 0; w < nhw; ++w) {
-                            unsigned msk = uniform(s_hmask[wave][w]);
-                            while (msk) {
-                                const int b = __builtin_ctz(msk);
-                                msk &= msk - 1;
-                                const int h = uniform((int)s_hcand[w * 32 + b]);
-                                if (h < PL.hole_begin || h >= PL.hole_end) continue;
+                        if ((layer_holes >> m) & 1u) {
+                            auto one_hole = [&](int h) {
                                 const Hole H = hol[h];
+                                if ((int)H.layer != m) return;
                                 const double c = chord_hole64(R64, H.x, (double)H.y1_raw + (double)p.fov2c,
                                                               (double)H.y2_raw + (double)p.fov2c, H.z, H.r);
                                 holes_len += c;
                                 att64 += (double)H.mu_t * c;
+                            };
+                            if (hol_fast) {
+                                for (int j = 0; j < n_hol; ++j) one_hole(uniform(s_hol[wave][j]));
+                            } else {
+                                for (int w = 0; w < nhw; ++w) {
+                                    unsigned msk = uniform(s_hmask[wave][w]);
+                                    while (msk) {
+                                        const int b = __builtin_ctz(msk);
+                                        msk &= msk - 1;
+                                        one_hole(uniform((int)s_hcand[w * 32 + b]));
+                                    }
+                                }
                             }
                         }
                         const double in_plate = (double)PL.thick / R64.ky;
-                        if ((in_plate - holes_len) >= (double)0.00001f) att64 += (double)PL.mu_t * (in_plate - holes_len);
+                        if ((in_plate - holes_len) >= (double)0.00001f)
+                            att64 += (double)PL.mu_t * (in_plate - holes_len);
                     }
                 }
                 // other crystals on the way (PESysMatGen.cu:687-725)
-                for (int w = 0; w < ncw; ++w) {
-                    unsigned msk = uniform(s_mask[wave][w]);
-                    while (msk) {
-                        const int b = __builtin_ctz(msk);
-                        msk &= msk - 1;
-                        const int k = uniform((int)s_cand[w * 32 + b]);
-                        const Crystal K = cry[k];
-                        float c;
-                        if (K.axis != 0.0f) {
-                            c = chord_through_c(R, K.cx, K.cy, K.cz, K.hx, K.hy, K.hz);
-                        } else {
-                            const float xPk = (xP - K.cx) * K.cr - (zP - K.cz) * K.sr, yPk = yP - K.cy;
-                            const float zPk = (xP - K.cx) * K.sr + (zP - K.cz) * K.cr;
-                            const float xQk = (xQ - K.cx) * K.cr - (zQ - K.cz) * K.sr, yQk = yQ - K.cy;
-                            const float zQk = (xQ - K.cx) * K.sr + (zQ - K.cz) * K.cr;
-                            const Ray Rk = make_ray(xPk, yPk, zPk, xQk, yQk, zQk);
-                            c = chord_through(Rk, -K.hx, -K.hy, -K.hz, K.hx, K.hy, K.hz);
+                if (att_fast) {
+                    for (int j = 0; j < n_att; ++j) {
+                        const Att A = s_att[wave][j];
+                        const int k = uniform(A.k);
+                        const bool rotated = k < 0;
+                        att += crystal_chord(R, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2,
+                                             rotated) * A.mu;
+                    }
+                } else {
+                    for (int w = 0; w < ncw; ++w) {
+                        unsigned msk = uniform(s_mask[wave][w]);
+                        while (msk) {
+                            const int b = __builtin_ctz(msk);
+                            msk &= msk - 1;
+                            const int k = uniform((int)s_cand[w * 32 + b]);
+                            const Crystal K = cry[k];
+                            att += crystal_chord(R, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy, K.cy + K.hy,
+                                                 K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) * K.mu_t;
                         }
-                        att += c * K.mu_t;
                     }
                 }
-                // path inside crystal d before the sub-cell (PESysMatGen.cu:727-746)
-                const float ux1 = ((float)nx / fcx - 0.5f) * wD, ux2 = (((float)nx + 1.0f) / fcx - 0.5f) * wD;
-                const float uy1 = ((float)ny / fcy - 0.5f) * tD, uy2 = (((float)ny + 1.0f) / fcy - 0.5f) * tD;
-                const float uz1 = ((float)nz / fcz - 0.5f) * hD, uz2 = (((float)nz + 1.0f) / fcz - 0.5f) * hD;
-                const Ray Rs = make_ray(xPs, yPs, zPs, xDs, yDs, zDs);
-                const float a1 = chord_open(Rs, -D.hx, -D.hy, -D.hz, D.hx, D.hy, D.hz);
-                const float a2 = chord_open(Rs, ux1, uy1, uz1, ux2, uy2, uz2);
+                // path inside crystal d before the sub-cell, and absorption in the
+                // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
+                const float ux1 = ((float)nx * rcx - 0.5f) * wD, ux2 = (((float)nx + 1.0f) * rcx - 0.5f) * wD;
+                const float uy1 = ((float)ny * rcy - 0.5f) * tD, uy2 = (((float)ny + 1.0f) * rcy - 0.5f) * tD;
+                const float uz1 = ((float)nz * rcz - 0.5f) * hD, uz2 = (((float)nz + 1.0f) * rcz - 0.5f) * hD;
+                const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
+                const float a1 = open_q(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz);
+                const float a2 = open_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
                 att += (a1 - a2) * D.mu_t;
                 const float eff = expf(-(float)(att64 + (double)att));
-                // absorption in the sub-cell along the extended ray (PESysMatGen.cu:749-759)
-                const float l_abs = chord_absorb(xPs, yPs, zPs, xDs, yDs, zDs, dist, ux1, uy1, uz1, ux2, uy2, uz2);
-                const float absorb = -expm1f(-l_abs * D.mu_t) * D.mu_pe / D.mu_t;
+                const float l_abs = absorb_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
+                const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
                 const float contrib = eff * solid * absorb;
                 if (!blocked) acc += (double)contrib;
             }

@@ -1,14 +1,20 @@
 // Device-side ray / solid primitives for gfx950.
 //
 // Semantics follow the reference device functions (file:line below); the
-// arithmetic is restructured for the GPU: everything that depends only on the
-// sub-ray (length, reciprocal directions, p * inv) is computed once per
-// sub-ray and reused for every box, so a slab test costs six FMAs plus
-// min/max instead of a square root and three divisions.
+// arithmetic is restructured for the GPU:
+//   * everything that depends only on the sub-ray (length, reciprocal
+//     directions, the parallel-axis pattern) is computed once per sub-ray and
+//     reused for every box it is tested against;
+//   * reciprocals / square roots use the hardware v_rcp_f32 / v_sqrt_f32
+//     (1 ulp) instead of the IEEE division / square-root sequences;
+//   * paths inside the detecting crystal are measured from the sub-point Q
+//     (which lies inside both the crystal and its sub-cell), so they are
+//     sub-millimetre numbers instead of differences of ~100 mm distances.
 //
-//   chord_through  <- length_box_ray        PESysMatGen.cu:19-151
-//   chord_open     <- length_box_ray_inside PESysMatGen.cu:153-307
-//   chord_hole64   <- length_cylinder_ray   PESysMatGen.cu:309-373 (in FP64)
+//   chord_world / chord_special  <- length_box_ray        PESysMatGen.cu:19-151
+//   open_q                       <- length_box_ray_inside PESysMatGen.cu:153-307
+//   absorb_q                     <- length_box_ray on the 1000 mm extension, PESysMatGen.cu:749-757
+//   chord_hole64                 <- length_cylinder_ray   PESysMatGen.cu:309-373 (in FP64)
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -16,193 +22,158 @@ namespace sysmat {
 
 constexpr float kEps = 0.001f;  // ray degeneracy / minimum chord (PESysMatGen.cu:22)
 
-// A segment P -> Q prepared for repeated slab tests.
-struct Ray {
-    float px, py, pz;
-    float dx, dy, dz;   // Q - P
-    float t;            // |Q - P|
-    float ix, iy, iz;   // t / d per axis (only meaningful on non-parallel axes)
-    float qx, qy, qz;   // end point (for the along-axis special cases)
-    int pat;            // bit 0/1/2: |d| < eps on x/y/z
-};
-
-__device__ inline Ray make_ray(float px, float py, float pz, float qx, float qy, float qz) {
-    Ray r;
-    r.px = px; r.py = py; r.pz = pz;
-    r.qx = qx; r.qy = qy; r.qz = qz;
-    r.dx = qx - px; r.dy = qy - py; r.dz = qz - pz;
-    r.t = __fsqrt_rn(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz);
-    r.pat = (fabsf(r.dx) < kEps ? 1 : 0) | (fabsf(r.dy) < kEps ? 2 : 0) | (fabsf(r.dz) < kEps ? 4 : 0);
-    r.ix = __fdiv_rn(r.t, r.dx);
-    r.iy = __fdiv_rn(r.t, r.dy);
-    r.iz = __fdiv_rn(r.t, r.dz);
-    return r;
-}
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
 __device__ inline bool inside_closed(float v, float a, float b) { return v >= a && v <= b; }
 
-// Slab interval along one axis for box [b1, b2], starting point p.
-__device__ inline void slab_iv(float p, float inv, float b1, float b2, float& lo, float& hi) {
-    float a = (b1 - p) * inv, b = (b2 - p) * inv;
+// A segment P -> Q prepared for repeated slab tests (world frame).
+struct WRay {
+    float px, py, pz;
+    float dx, dy, dz;   // Q - P
+    float ix, iy, iz;   // t / d
+    float t;            // |Q - P|
+    int pat;            // bit 0/1/2: |d| < eps on x/y/z
+};
+
+__device__ inline WRay make_wray(float px, float py, float pz, float qx, float qy, float qz) {
+    WRay r;
+    r.px = px; r.py = py; r.pz = pz;
+    r.dx = qx - px; r.dy = qy - py; r.dz = qz - pz;
+    r.t = fsqrt(r.dx * r.dx + r.dy * r.dy + r.dz * r.dz);
+    r.pat = (fabsf(r.dx) < kEps ? 1 : 0) | (fabsf(r.dy) < kEps ? 2 : 0) | (fabsf(r.dz) < kEps ? 4 : 0);
+    r.ix = r.t * frcp(r.dx);
+    r.iy = r.t * frcp(r.dy);
+    r.iz = r.t * frcp(r.dz);
+    return r;
+}
+
+// Branches of length_box_ray for rays parallel to one, two or three axes.
+// Coordinates are relative to the box frame the caller chose.
+__device__ inline float chord_special(float px, float py, float pz, float qx, float qy, float qz, float ix,
+                                      float iy, float iz, float t, int pat, float x1, float y1, float z1, float x2,
+                                      float y2, float z2) {
+    if (pat == 7) return 0.0f;
+    if (pat == 3) {  // along z
+        return (inside_closed(px, x1, x2) && inside_closed(py, y1, y2) &&
+                ((pz <= z1 && qz >= z2) || (qz <= z1 && pz >= z2))) ? fabsf(z2 - z1) : 0.0f;
+    }
+    if (pat == 6) {  // along x
+        return (inside_closed(pz, z1, z2) && inside_closed(py, y1, y2) &&
+                ((px <= x1 && qx >= x2) || (qx <= x1 && px >= x2))) ? fabsf(x2 - x1) : 0.0f;
+    }
+    if (pat == 5) {  // along y
+        return (inside_closed(px, x1, x2) && inside_closed(pz, z1, z2) &&
+                ((py <= y1 && qy >= y2) || (qy <= y1 && py >= y2))) ? fabsf(y2 - y1) : 0.0f;
+    }
+    // one parallel axis: strictly inside that axis' range, slabs on the other two
+    if ((pat == 1 && (px >= x2 || px <= x1)) || (pat == 2 && (py >= y2 || py <= y1)) ||
+        (pat == 4 && (pz >= z2 || pz <= z1)))
+        return 0.0f;
+    float lo = -INFINITY, hi = INFINITY, a, b;
+    if (pat != 1) { a = (x1 - px) * ix; b = (x2 - px) * ix; lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b)); }
+    if (pat != 2) { a = (y1 - py) * iy; b = (y2 - py) * iy; lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b)); }
+    if (pat != 4) { a = (z1 - pz) * iz; b = (z2 - pz) * iz; lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b)); }
+    const float c = hi - lo;
+    return (c >= kEps && lo > kEps && hi < t) ? c : 0.0f;
+}
+
+// length_box_ray for box [x1,x2] x [y1,y2] x [z1,z2] in the ray's frame: the
+// chord counts only when the segment enters and leaves strictly between its
+// end points (tmax - tmin >= eps, tmin > eps, tmax < t).
+__device__ inline float chord_world(const WRay& r, float x1, float x2, float y1, float y2, float z1, float z2) {
+    float a = (x1 - r.px) * r.ix, b = (x2 - r.px) * r.ix;
+    float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = (y1 - r.py) * r.iy; b = (y2 - r.py) * r.iy;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    a = (z1 - r.pz) * r.iz; b = (z2 - r.pz) * r.iz;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    const float c = hi - lo;
+    return (c >= kEps && lo > kEps && hi < r.t) ? c : 0.0f;
+}
+
+// Sub-ray in the detecting crystal's frame, parameterised from Q (inside the
+// crystal): s = (param from P) - |Q - P|.
+struct QRay {
+    float px, py, pz;   // P in the crystal frame
+    float qx, qy, qz;   // Q in the crystal frame
+    float ix, iy, iz;   // |u| / u, u = Q - P
+    float t;            // |u|
+    int pat;            // |u| < eps per axis
+    int pat_e;          // same test on the 1000 mm extension (PESysMatGen.cu:749-752)
+    float t_ext;        // length of the extension beyond Q
+};
+
+__device__ inline QRay make_qray(float px, float py, float pz, float qx, float qy, float qz, float dist_world) {
+    QRay r;
+    r.px = px; r.py = py; r.pz = pz;
+    r.qx = qx; r.qy = qy; r.qz = qz;
+    const float ux = qx - px, uy = qy - py, uz = qz - pz;
+    r.t = fsqrt(ux * ux + uy * uy + uz * uz);
+    r.pat = (fabsf(ux) < kEps ? 1 : 0) | (fabsf(uy) < kEps ? 2 : 0) | (fabsf(uz) < kEps ? 4 : 0);
+    r.ix = r.t * frcp(ux);
+    r.iy = r.t * frcp(uy);
+    r.iz = r.t * frcp(uz);
+    // extended end point E = Q + 1000 (Q - P) / dist, as the reference builds it
+    const float k = 1000.0f * frcp(dist_world);
+    const float ex = (qx + k * ux) - px, ey = (qy + k * uy) - py, ez = (qz + k * uz) - pz;
+    r.pat_e = (fabsf(ex) < kEps ? 1 : 0) | (fabsf(ey) < kEps ? 2 : 0) | (fabsf(ez) < kEps ? 4 : 0);
+    r.t_ext = k * r.t;
+    return r;
+}
+
+// Slab interval (relative to Q) of the box on one axis.
+__device__ inline void qslab(float q, float inv, float b1, float b2, float& lo, float& hi) {
+    const float a = (b1 - q) * inv, b = (b2 - q) * inv;
     lo = fminf(a, b);
     hi = fmaxf(a, b);
 }
 
-// Rare branches of length_box_ray: rays parallel to one, two or three axes.
-__device__ __noinline__ float chord_through_special(const Ray& r, float x1, float y1, float z1, float x2, float y2,
-                                                    float z2) {
-    const int pat = r.pat;
-    if (pat == 7) return 0.0f;
-    if (pat == 3) {  // along z
-        if (inside_closed(r.px, x1, x2) && inside_closed(r.py, y1, y2) &&
-            ((r.pz <= z1 && r.qz >= z2) || (r.qz <= z1 && r.pz >= z2)))
-            return fabsf(z2 - z1);
-        return 0.0f;
-    }
-    if (pat == 6) {  // along x
-        if (inside_closed(r.pz, z1, z2) && inside_closed(r.py, y1, y2) &&
-            ((r.px <= x1 && r.qx >= x2) || (r.qx <= x1 && r.px >= x2)))
-            return fabsf(x2 - x1);
-        return 0.0f;
-    }
-    if (pat == 5) {  // along y
-        if (inside_closed(r.px, x1, x2) && inside_closed(r.pz, z1, z2) &&
-            ((r.py <= y1 && r.qy >= y2) || (r.qy <= y1 && r.py >= y2)))
-            return fabsf(y2 - y1);
-        return 0.0f;
-    }
-    // exactly one parallel axis: the start point must be strictly inside that
-    // axis' range, the other two axes are slab-tested
-    float lo = -INFINITY, hi = INFINITY, a, b;
-    if (pat == 1) {
-        if (r.px >= x2 || r.px <= x1) return 0.0f;
-    } else {
-        slab_iv(r.px, r.ix, x1, x2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    if (pat == 2) {
-        if (r.py >= y2 || r.py <= y1) return 0.0f;
-    } else {
-        slab_iv(r.py, r.iy, y1, y2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    if (pat == 4) {
-        if (r.pz >= z2 || r.pz <= z1) return 0.0f;
-    } else {
-        slab_iv(r.pz, r.iz, z1, z2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    return ((hi - lo) >= kEps && lo > kEps && hi < r.t) ? hi - lo : 0.0f;
-}
-
-// length_box_ray for box [x1..x2] x [y1..y2] x [z1..z2]: the chord counts only
-// when the segment enters and leaves strictly between its end points.
-__device__ inline float chord_through(const Ray& r, float x1, float y1, float z1, float x2, float y2, float z2) {
-    if (r.pat != 0) return chord_through_special(r, x1, y1, z1, x2, y2, z2);
-    float ax, bx, ay, by, az, bz;
-    slab_iv(r.px, r.ix, x1, x2, ax, bx);
-    slab_iv(r.py, r.iy, y1, y2, ay, by);
-    slab_iv(r.pz, r.iz, z1, z2, az, bz);
-    float lo = fmaxf(fmaxf(ax, ay), az);
-    float hi = fminf(fminf(bx, by), bz);
-    return ((hi - lo) >= kEps && lo > kEps && hi < r.t) ? hi - lo : 0.0f;
-}
-
-// Same test for a box given as centre +- half extent; the along-axis branches
-// compare the start point in the box frame the way the reference does.
-__device__ inline float chord_through_c(const Ray& r, float cx, float cy, float cz, float hx, float hy, float hz) {
-    if (r.pat != 0) {
-        Ray s = r;
-        s.px = r.px - cx; s.py = r.py - cy; s.pz = r.pz - cz;
-        s.qx = r.qx - cx; s.qy = r.qy - cy; s.qz = r.qz - cz;
-        return chord_through_special(s, -hx, -hy, -hz, hx, hy, hz);
-    }
-    float ax, bx, ay, by, az, bz;
-    slab_iv(r.px, r.ix, cx - hx, cx + hx, ax, bx);
-    slab_iv(r.py, r.iy, cy - hy, cy + hy, ay, by);
-    slab_iv(r.pz, r.iz, cz - hz, cz + hz, az, bz);
-    float lo = fmaxf(fmaxf(ax, ay), az);
-    float hi = fminf(fminf(bx, by), bz);
-    return ((hi - lo) >= kEps && lo > kEps && hi < r.t) ? hi - lo : 0.0f;
-}
-
-// Closing rules of length_box_ray_inside (end point may be inside the box).
-__device__ inline float finish_open(float lo, float hi, float t) {
-    if ((hi - lo) < kEps) return 0.0f;
-    if (lo >= t) return 0.0f;
-    if (hi <= kEps) return 0.0f;
-    if ((hi >= t) && (lo > kEps)) return t - lo;
-    if ((lo <= kEps) && (hi <= t)) return hi;
-    if ((lo <= kEps) && (hi >= t)) return t;
-    return hi - lo;
-}
-
-// length_box_ray_inside: path of P -> Q inside the box when Q may lie inside.
-__device__ inline float chord_open(const Ray& r, float x1, float y1, float z1, float x2, float y2, float z2) {
-    if (r.pat & 2) return 0.0f;  // |dy| < 0.001 returns 0 first
-    if (r.pat == 5) {            // along y
+// length_box_ray_inside for a box that contains Q: path of P -> Q inside it.
+__device__ inline float open_q(const QRay& r, float x1, float x2, float y1, float y2, float z1, float z2) {
+    if (r.pat & 2) return 0.0f;  // |dy| < 0.001 -> 0 (PESysMatGen.cu:156)
+    if ((r.pat & 5) == 5) {      // along y
         if (inside_closed(r.px, x1, x2) && inside_closed(r.pz, z1, z2)) {
             if (r.py < r.qy) return r.qy - y1;
             if (r.py > r.qy) return y2 - r.qy;
         }
         return 0.0f;
     }
-    float lo = -INFINITY, hi = INFINITY, a, b;
-    if (r.pat & 1) {
-        if (r.px >= x2 || r.px <= x1) return 0.0f;
-    } else {
-        slab_iv(r.px, r.ix, x1, x2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    slab_iv(r.py, r.iy, y1, y2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    if (r.pat & 4) {
-        if (r.pz >= z2 || r.pz <= z1) return 0.0f;
-    } else {
-        slab_iv(r.pz, r.iz, z1, z2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    return finish_open(lo, hi, r.t);
+    if (((r.pat & 1) && (r.px >= x2 || r.px <= x1)) || ((r.pat & 4) && (r.pz >= z2 || r.pz <= z1))) return 0.0f;
+    float sl, sh, a, b;
+    qslab(r.qy, r.iy, y1, y2, sl, sh);
+    if (!(r.pat & 1)) { qslab(r.qx, r.ix, x1, x2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b); }
+    if (!(r.pat & 4)) { qslab(r.qz, r.iz, z1, z2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b); }
+    // closing rules of the reference with lo = t + sl, hi = t + sh
+    if ((sh - sl) < kEps) return 0.0f;
+    if (sl >= 0.0f) return 0.0f;
+    if (r.t + sh <= kEps) return 0.0f;
+    const bool lo_in = (r.t + sl) > kEps;
+    if (sh >= 0.0f && lo_in) return -sl;
+    if (!lo_in && sh <= 0.0f) return r.t + sh;
+    if (!lo_in && sh >= 0.0f) return r.t;
+    return sh - sl;
 }
 
-// length_box_ray of the ray P -> Q extended 1000 mm past Q, through a box
-// that contains Q (the sub-cell).  The reference evaluates it in distances
-// from P (~1000 mm, FP32 ulp ~1e-4 mm against a ~0.2 mm chord); here the slab
-// distances are taken from Q, which keeps them sub-millimetre and exact to
-// ~1e-8 mm.  Branch rules are those of length_box_ray on the extended segment.
-__device__ inline float chord_absorb(float px, float py, float pz, float qx, float qy, float qz, float dist,
-                                     float x1, float y1, float z1, float x2, float y2, float z2) {
-    const float ext = 1000.0f;
-    float ex = qx + ext * (qx - px) / dist;
-    float ey = qy + ext * (qy - py) / dist;
-    float ez = qz + ext * (qz - pz) / dist;
-    float dx = ex - px, dy = ey - py, dz = ez - pz;
-    int pat = (fabsf(dx) < kEps ? 1 : 0) | (fabsf(dy) < kEps ? 2 : 0) | (fabsf(dz) < kEps ? 4 : 0);
+// length_box_ray of P -> E (E = Q extended 1000 mm) through a box that
+// contains Q, measured from Q.
+__device__ inline float absorb_q(const QRay& r, float x1, float x2, float y1, float y2, float z1, float z2) {
+    const int pat = r.pat_e;
     if (pat == 7) return 0.0f;
     if (pat == 3 || pat == 5 || pat == 6) {
-        Ray s;
-        s.px = px; s.py = py; s.pz = pz; s.qx = ex; s.qy = ey; s.qz = ez; s.pat = pat;
-        return chord_through_special(s, x1, y1, z1, x2, y2, z2);
+        const float f = r.t_ext * frcp(r.t);  // E = Q + f (Q - P)
+        return chord_special(r.px, r.py, r.pz, r.qx + f * (r.qx - r.px), r.qy + f * (r.qy - r.py),
+                             r.qz + f * (r.qz - r.pz), 0.0f, 0.0f, 0.0f, 0.0f, pat, x1, y1, z1, x2, y2, z2);
     }
-    // direction of Q - P, normalised
-    float ux = qx - px, uy = qy - py, uz = qz - pz;
-    float tq = __fsqrt_rn(ux * ux + uy * uy + uz * uz);
-    float lo = -INFINITY, hi = INFINITY, a, b;
-    if (pat & 1) {
-        if (px >= x2 || px <= x1) return 0.0f;
-    } else {
-        float inv = __fdiv_rn(tq, ux);
-        slab_iv(qx, inv, x1, x2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    if (pat & 2) {
-        if (py >= y2 || py <= y1) return 0.0f;
-    } else {
-        float inv = __fdiv_rn(tq, uy);
-        slab_iv(qy, inv, y1, y2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    if (pat & 4) {
-        if (pz >= z2 || pz <= z1) return 0.0f;
-    } else {
-        float inv = __fdiv_rn(tq, uz);
-        slab_iv(qz, inv, z1, z2, a, b); lo = fmaxf(lo, a); hi = fminf(hi, b);
-    }
-    // entry strictly after P, exit strictly before the extended end
-    float t_end = ext * tq / dist;
-    return ((hi - lo) >= kEps && tq + lo > kEps && hi < t_end) ? hi - lo : 0.0f;
+    if ((pat == 1 && (r.px >= x2 || r.px <= x1)) || (pat == 2 && (r.py >= y2 || r.py <= y1)) ||
+        (pat == 4 && (r.pz >= z2 || r.pz <= z1)))
+        return 0.0f;
+    float sl = -INFINITY, sh = INFINITY, a, b;
+    if (pat != 1) { qslab(r.qx, r.ix, x1, x2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b); }
+    if (pat != 2) { qslab(r.qy, r.iy, y1, y2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b); }
+    if (pat != 4) { qslab(r.qz, r.iz, z1, z2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b); }
+    const float c = sh - sl;
+    return (c >= kEps && r.t + sl > kEps && sh < r.t_ext) ? c : 0.0f;
 }
 
 // Per-sub-ray FP64 data of the collimator path.  The end points themselves are

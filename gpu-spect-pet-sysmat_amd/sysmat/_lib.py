@@ -12,7 +12,7 @@ import os
 import numpy as np
 
 PKG_DIR = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-LIB_PATH = os.path.join(PKG_DIR, "libsysmat_hip.so")
+LIB_PATH = os.environ.get("SYSMAT_LIB") or os.path.join(PKG_DIR, "libsysmat_hip.so")
 
 SM_OK = 0
 SM_E_ARG = -1
