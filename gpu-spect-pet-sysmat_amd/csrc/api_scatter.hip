@@ -1,0 +1,509 @@
+// C ABI and host orchestration of the Compton-scatter matrices.
+//
+// Replaces scatter() (ScatterGen_RayTracing_CircularHole/scatter.cu:2013-2378 and
+// the ScatterGen_Collimator / ScatterGen_Crystal variants): bitmaps are built
+// on the GPU (or read from the GeometryRelationShip_* files), per-pair
+// constants are prepared once, and the per-voxel kernel runs over detector
+// blocks.  Host-memory entry points mirror the reference signatures; plans keep
+// everything resident in HBM for repeated launches.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/sysmat.h"
+#include "common.hpp"
+#include "scatter.hpp"
+
+namespace sysmat {
+
+// Reference KN normalisation: h * sum_{i=0..n} f(i h), n = (int)(pi_f / h), in
+// double with the reference's float constants (scatter.cu:1169).
+static double kn_total_ref(double e0) {
+    const double h = (double)0.01f, top = (double)(float)M_PI;
+    const int n = (int)((top - 0.0) / h);
+    double s = 0.0;
+    for (int i = 0; i <= n; ++i) {
+        const double th = 0.0 + (double)(float)i * h;
+        const double c = std::cos(th), a = e0 / 511.0;
+        const double f1 = a * (1.0 - c), f2 = 1.0 + c * c;
+        s += f2 / (1.0 + f1) / (1.0 + f1) * (1.0 + f1 * f1 / f2 / (1.0 + f1)) * std::sin(th) * h;
+    }
+    return s;
+}
+
+// Scatter angles (theta_lo, theta_hi) where the 2-sigma energy-window test
+// of scatter.cu:1263 can pass (E' is monotone in theta); a small margin keeps
+// the pruning conservative.
+static void window_band(double e0, double res, double lo, double hi, double& th_lo, double& th_hi) {
+    auto ep = [&](double th) { return e0 / (1.0 + (e0 / 511.0) * (1.0 - std::cos(th))); };
+    auto fail_low = [&](double th) {
+        const double e = ep(th), k = 2.0 * res * std::sqrt(e / e0) / (double)2.35482f;
+        return e * (1.0 + k) <= lo;
+    };
+    auto fail_high = [&](double th) {
+        const double e = ep(th), k = 2.0 * res * std::sqrt(e / e0) / (double)2.35482f;
+        return e * (1.0 - k) >= hi;
+    };
+    // fail_low holds for large theta, fail_high for small theta (E' decreases with theta)
+    double a = 0.0, b = M_PI;
+    if (fail_low(0.0)) { th_lo = 1.0; th_hi = -1.0; return; }  // nothing passes
+    if (!fail_low(M_PI)) {
+        th_hi = M_PI;
+    } else {
+        for (int i = 0; i < 80; ++i) { const double m = 0.5 * (a + b); (fail_low(m) ? b : a) = m; }
+        th_hi = b;
+    }
+    if (!fail_high(0.0)) {
+        th_lo = 0.0;
+    } else {
+        a = 0.0; b = M_PI;
+        for (int i = 0; i < 80; ++i) { const double m = 0.5 * (a + b); (fail_high(m) ? a : b) = m; }
+        th_lo = a;
+    }
+    th_lo = std::max(0.0, th_lo - 1e-6);
+    th_hi = std::min(M_PI, th_hi + 1e-6);
+}
+
+struct ScatterGeom {
+    int device = 0;
+    Tables T;
+    DeviceBuf<Crystal> cry;
+    DeviceBuf<Hole> hol;
+    DeviceBuf<ScRow> rows;
+    std::vector<ScRow> hrows;
+    ScParams base{};
+    int n_holes0 = 0;
+};
+
+static int build_geom(const float* col, const float* det, const float* img, const float* phy, int device,
+                      int rotation, ScatterGeom& G) {
+    G.device = device;
+    std::string err = build_tables(col, det, img[11], G.T);
+    if (!err.empty()) return set_error(SM_E_ARG, err);
+    int rc;
+    if ((rc = G.cry.upload(G.T.crystals)) < 0) return rc;
+    if ((rc = G.hol.upload(G.T.holes)) < 0) return rc;
+    ScParams& p = G.base;
+    p = ScParams{};
+    p.X = (int)std::floor(img[0] + 0.001f);
+    p.Y = (int)std::floor(img[1] + 0.001f);
+    p.Z = (int)std::floor(img[2] + 0.001f);
+    if (p.X <= 0 || p.Y <= 0 || p.Z <= 0) return set_error(SM_E_ARG, "image voxel counts must be positive");
+    p.wx = img[3]; p.wy = img[4]; p.wz = img[5];
+    p.sx = img[8]; p.sy = img[9]; p.sz = img[10];
+    const double ang = (double)(float)rotation * (double)img[7];
+    p.rc = std::cos(ang);
+    p.rs = std::sin(ang);
+    p.e0 = phy[7];
+    if (!(p.e0 > 0.0)) return set_error(SM_E_ARG, "photopeak energy Phys[7] must be positive");
+    p.kn_total = kn_total_ref(p.e0);
+    p.h = (double)0.01f;
+    p.n_det = G.T.n_det;
+    p.V = (long long)p.X * p.Y * p.Z;
+    p.fov2c = img[11];
+    G.n_holes0 = (int)std::floor(col[10] + 0.000001f);
+    if (G.n_holes0 > G.T.n_holes) G.n_holes0 = G.T.n_holes;
+    p.w_col = (col[15] != 0.0f) ? (double)col[17] / (double)col[15] : 0.0;
+    const bool use_compton = (int)std::floor(phy[0] + 0.5f) == 1;
+    const bool same_window = (int)std::floor(phy[4] + 0.5f) > 0;
+    G.hrows.resize(G.T.n_det);
+    for (int r = 0; r < G.T.n_det; ++r) {
+        const float* D = det + 12 * r;
+        ScRow& R = G.hrows[r];
+        R = ScRow{};
+        R.cx = D[1];
+        R.cy = (double)D[2] + (double)img[11];
+        R.cz = D[3];
+        R.crn = std::cos(-(double)D[11]);
+        R.srn = std::sin(-(double)D[11]);
+        R.hx = 0.5 * (double)D[4];
+        R.hy = 0.5 * (double)D[5];
+        R.hz = 0.5 * (double)D[6];
+        R.res = D[10];
+        if (same_window) {
+            R.lo = phy[5];
+            R.hi = phy[6];
+        } else {
+            R.lo = (1.0 - R.res / 2.0) * p.e0;
+            R.hi = (1.0 + R.res / 2.0) * p.e0;
+        }
+        R.mu_pe = D[8];
+        R.mu_c = D[9];
+        const double w = D[4], h = D[6], t = D[5];
+        R.rdet = std::sqrt(w * w / 1.0 / 1.0 + h * h / 1.0 / 1.0 + t * t / 1.0 / 1.0) / 2.0;
+        window_band(p.e0, R.res, R.lo, R.hi, R.th_lo, R.th_hi);
+        R.use = (use_compton && D[7] > 0.01f && D[8] != 0.0f) ? 1.0f : 0.0f;
+    }
+    return G.rows.upload(G.hrows);
+}
+
+long long crystal_words(int n) { return ((long long)n * (n + 1) / 2 * n + 31) / 32; }
+long long collimator_words(int nh, int n) { return ((long long)n * n * nh + 31) / 32; }
+
+// Run prep + accumulation for detector rows [det_begin, det_begin + count) in
+// blocks that keep the pair table under `pair_budget` bytes.
+static int run_scatter(ScatterGeom& G, int mode, const unsigned* bits_dev, const float* pe_dev, int det_begin,
+                       int det_count, float* dst_dev, long long row_stride, hipStream_t s) {
+    ScParams p = G.base;
+    p.mode = mode;
+    p.max_sites = (mode == 0) ? G.T.n_det : std::max(1, G.n_holes0);
+    if (mode != 0 && G.n_holes0 <= 0) {  // no layer-0 holes: the collimator term is zero
+        for (int r = 0; r < det_count; ++r)
+            HIP_TRY(hipMemsetAsync(dst_dev + (long long)r * row_stride, 0, sizeof(float) * p.V, s));
+        return SM_OK;
+    }
+    const long long pair_budget = 2048ll << 20;  // 2 GiB of pair records per block
+    int block = (int)std::max(4ll, std::min<long long>(det_count, pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
+    block = (block + 3) / 4 * 4;
+    DeviceBuf<ScPair> pairs;
+    DeviceBuf<int> counts;
+    int rc;
+    if ((rc = pairs.alloc((size_t)block * p.max_sites)) < 0) return rc;
+    if ((rc = counts.alloc(block)) < 0) return rc;
+    for (int b0 = 0; b0 < det_count; b0 += block) {
+        const int cnt = std::min(block, det_count - b0);
+        p.det_begin = det_begin + b0;
+        p.det_count = cnt;
+        p.row_stride = row_stride;
+        p.n_tiles = (int)((p.V + 63) / 64);
+        p.n_rgroups = (cnt + 3) / 4;
+        p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
+        HIP_TRY(launch_scatter_prep(p, G.rows.ptr, G.cry.ptr, G.hol.ptr, bits_dev, pairs.ptr, counts.ptr, s));
+        HIP_TRY(launch_scatter(p, G.rows.ptr, pairs.ptr, counts.ptr, pe_dev, dst_dev + (long long)b0 * row_stride, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return SM_OK;
+}
+
+static int read_words(const char* fname, std::vector<unsigned>& w) {
+    FILE* f = std::fopen(fname, "rb");
+    if (!f) return set_error(SM_E_IO, std::string("cannot open ") + fname);
+    size_t got = std::fread(w.data(), sizeof(unsigned), w.size(), f);
+    std::fclose(f);
+    if (got != w.size()) return set_error(SM_E_IO, std::string("short geometry relationship file ") + fname);
+    return SM_OK;
+}
+
+static int write_words(const char* fname, const std::vector<unsigned>& w) {
+    FILE* f = std::fopen(fname, "wb+");
+    if (!f) return set_error(SM_E_IO, std::string("cannot create ") + fname);
+    size_t put = std::fwrite(w.data(), sizeof(unsigned), w.size(), f);
+    std::fclose(f);
+    if (put != w.size()) return set_error(SM_E_IO, std::string("short write of ") + fname);
+    return SM_OK;
+}
+
+// Device bitmap for `mode` (0 crystal, else collimator): computed, or loaded from
+// the file when `compute` is false; optionally written to the file after computing.
+static int make_bitmap(ScatterGeom& G, int mode, bool compute, const char* fname, bool write_file,
+                       DeviceBuf<unsigned>& bits, hipStream_t s) {
+    const int n = G.T.n_det;
+    const long long nw = (mode == 0) ? crystal_words(n) : collimator_words(G.n_holes0, n);
+    int rc;
+    if ((rc = bits.alloc((size_t)std::max(1ll, nw))) < 0) return rc;
+    if (compute) {
+        HIP_TRY(hipMemsetAsync(bits.ptr, 0, sizeof(unsigned) * (size_t)std::max(1ll, nw), s));
+        if (mode == 0)
+            HIP_TRY(launch_bitmap_crystal(G.cry.ptr, n, bits.ptr, s));
+        else
+            HIP_TRY(launch_bitmap_collimator(G.hol.ptr, G.n_holes0, G.cry.ptr, n, bits.ptr, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (write_file && fname && nw > 0) {
+            std::vector<unsigned> h((size_t)nw);
+            HIP_TRY(hipMemcpy(h.data(), bits.ptr, sizeof(unsigned) * (size_t)nw, hipMemcpyDeviceToHost));
+            if ((rc = write_words(fname, h)) < 0) return rc;
+        }
+    } else {
+        std::vector<unsigned> h((size_t)std::max(1ll, nw), 0u);
+        if (nw > 0 && (rc = read_words(fname, h)) < 0) return rc;
+        HIP_TRY(hipMemcpy(bits.ptr, h.data(), sizeof(unsigned) * h.size(), hipMemcpyHostToDevice));
+    }
+    return SM_OK;
+}
+
+}  // namespace sysmat
+
+using namespace sysmat;
+
+struct SmScatterPlan {
+    ScatterGeom G;
+    DeviceBuf<unsigned> bits_crystal, bits_collimator;
+    int variant = 0;
+};
+
+extern "C" {
+
+long long sm_crystal_bitmap_words(int n_det) { return crystal_words(n_det); }
+long long sm_collimator_bitmap_words(int n_holes0, int n_det) { return collimator_words(n_holes0, n_det); }
+
+int sm_bitmap_crystal(const float* det, unsigned* words, long long n_words, int device) {
+    clear_error();
+    if (!det || !words) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(device);
+    if (rc < 0) return rc;
+    float col0[110] = {0};
+    float img0[32] = {0};
+    float phy0[10] = {0};
+    phy0[7] = 140.0f;
+    img0[0] = img0[1] = img0[2] = 1.0f;
+    ScatterGeom G;
+    if ((rc = build_geom(col0, det, img0, phy0, device, 0, G)) < 0) return rc;
+    if (n_words < crystal_words(G.T.n_det)) return set_error(SM_E_ARG, "bitmap buffer too small");
+    DeviceBuf<unsigned> bits;
+    if ((rc = make_bitmap(G, 0, true, nullptr, false, bits, nullptr)) < 0) return rc;
+    HIP_TRY(hipMemcpy(words, bits.ptr, sizeof(unsigned) * crystal_words(G.T.n_det), hipMemcpyDeviceToHost));
+    return SM_OK;
+}
+
+int sm_bitmap_collimator(const float* col, const float* det, unsigned* words, long long n_words, int device) {
+    clear_error();
+    if (!col || !det || !words) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(device);
+    if (rc < 0) return rc;
+    float img0[32] = {0};
+    float phy0[10] = {0};
+    phy0[7] = 140.0f;
+    img0[0] = img0[1] = img0[2] = 1.0f;
+    ScatterGeom G;
+    if ((rc = build_geom(col, det, img0, phy0, device, 0, G)) < 0) return rc;
+    const long long nw = collimator_words(G.n_holes0, G.T.n_det);
+    if (n_words < nw) return set_error(SM_E_ARG, "bitmap buffer too small");
+    DeviceBuf<unsigned> bits;
+    if ((rc = make_bitmap(G, 1, true, nullptr, false, bits, nullptr)) < 0) return rc;
+    if (nw > 0) HIP_TRY(hipMemcpy(words, bits.ptr, sizeof(unsigned) * nw, hipMemcpyDeviceToHost));
+    return SM_OK;
+}
+
+int sm_scatter_plan_create(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                           int collimator_variant, const unsigned* crystal_words_host,
+                           const unsigned* collimator_words_host, int device, SmScatterPlan** plan) {
+    clear_error();
+    if (!det || !img || !phy || !plan) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(device);
+    if (rc < 0) return rc;
+    float col0[110] = {0};
+    auto* P = new SmScatterPlan();
+    P->variant = collimator_variant;
+    rc = build_geom(col ? col : col0, det, img, phy, device, rotation, P->G);
+    if (rc >= 0) {
+        const int n = P->G.T.n_det;
+        if (crystal_words_host) {
+            const long long nw = crystal_words(n);
+            if ((rc = P->bits_crystal.alloc(nw)) >= 0) {
+                hipError_t e = hipMemcpy(P->bits_crystal.ptr, crystal_words_host, sizeof(unsigned) * nw,
+                                         hipMemcpyHostToDevice);
+                if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "bitmap upload failed");
+            }
+        } else {
+            rc = make_bitmap(P->G, 0, true, nullptr, false, P->bits_crystal, nullptr);
+        }
+    }
+    if (rc >= 0 && col) {
+        const long long nw = collimator_words(P->G.n_holes0, P->G.T.n_det);
+        if (collimator_words_host) {
+            if ((rc = P->bits_collimator.alloc(std::max(1ll, nw))) >= 0 && nw > 0) {
+                hipError_t e = hipMemcpy(P->bits_collimator.ptr, collimator_words_host, sizeof(unsigned) * nw,
+                                         hipMemcpyHostToDevice);
+                if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "bitmap upload failed");
+            }
+        } else {
+            rc = make_bitmap(P->G, 1, true, nullptr, false, P->bits_collimator, nullptr);
+        }
+    }
+    if (rc < 0) {
+        delete P;
+        return rc;
+    }
+    *plan = P;
+    return SM_OK;
+}
+
+int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
+                            long long row_stride, void* stream) {
+    clear_error();
+    if (!plan || !pe_dev || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    const int n = plan->G.T.n_det;
+    if (det_begin < 0 || det_count <= 0 || det_begin + det_count > n)
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    int rc = use_device(plan->G.device);
+    if (rc < 0) return rc;
+    return run_scatter(plan->G, 0, plan->bits_crystal.ptr, pe_dev, det_begin, det_count, dst_dev,
+                       row_stride > 0 ? row_stride : plan->G.base.V, (hipStream_t)stream);
+}
+
+int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
+                               long long row_stride, void* stream) {
+    clear_error();
+    if (!plan || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    if (!plan->bits_collimator.ptr) return set_error(SM_E_ARG, "plan created without a collimator");
+    const int n = plan->G.T.n_det;
+    if (det_begin < 0 || det_count <= 0 || det_begin + det_count > n)
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    int rc = use_device(plan->G.device);
+    if (rc < 0) return rc;
+    return run_scatter(plan->G, plan->variant == 1 ? 2 : 1, plan->bits_collimator.ptr, nullptr, det_begin, det_count,
+                       dst_dev, row_stride > 0 ? row_stride : plan->G.base.V, (hipStream_t)stream);
+}
+
+int sm_scatter_plan_destroy(SmScatterPlan* plan) {
+    if (plan) {
+        use_device(plan->G.device);
+        delete plan;
+    }
+    return SM_OK;
+}
+
+// Host-memory convenience: crystal scatter rows of the N x V PE matrix `pe`.
+int sm_scatter_crystal(const float* det, const float* img, const float* phy, const float* pe,
+                       const unsigned* crystal_words_host, int rotation, int det_begin, int det_count, float* dst,
+                       int device) {
+    SmScatterPlan* plan = nullptr;
+    int rc = sm_scatter_plan_create(nullptr, det, img, phy, rotation, 0, crystal_words_host, nullptr, device, &plan);
+    if (rc < 0) return rc;
+    const long long V = plan->G.base.V, N = plan->G.T.n_det;
+    if (det_count <= 0) det_count = (int)N - det_begin;
+    DeviceBuf<float> dpe, dout;
+    if ((rc = dpe.alloc((size_t)(N * V))) >= 0 && (rc = dout.alloc((size_t)det_count * V)) >= 0) {
+        hipError_t e = hipMemcpy(dpe.ptr, pe, sizeof(float) * N * V, hipMemcpyHostToDevice);
+        if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "PE upload failed");
+        if (rc >= 0) rc = sm_scatter_plan_crystal(plan, dpe.ptr, det_begin, det_count, dout.ptr, V, nullptr);
+        if (rc >= 0) {
+            e = hipMemcpy(dst, dout.ptr, sizeof(float) * det_count * V, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "result download failed");
+        }
+    }
+    sm_scatter_plan_destroy(plan);
+    return rc;
+}
+
+int sm_scatter_collimator(const float* col, const float* det, const float* img, const float* phy,
+                          const unsigned* collimator_words_host, int rotation, int variant, int det_begin,
+                          int det_count, float* dst, int device) {
+    SmScatterPlan* plan = nullptr;
+    float* none = nullptr;
+    (void)none;
+    int rc = sm_scatter_plan_create(col, det, img, phy, rotation, variant, nullptr, collimator_words_host, device, &plan);
+    if (rc < 0) return rc;
+    const long long V = plan->G.base.V, N = plan->G.T.n_det;
+    if (det_count <= 0) det_count = (int)N - det_begin;
+    DeviceBuf<float> dout;
+    if ((rc = dout.alloc((size_t)det_count * V)) >= 0) {
+        rc = sm_scatter_plan_collimator(plan, det_begin, det_count, dout.ptr, V, nullptr);
+        if (rc >= 0) {
+            hipError_t e = hipMemcpy(dst, dout.ptr, sizeof(float) * det_count * V, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "result download failed");
+        }
+    }
+    sm_scatter_plan_destroy(plan);
+    return rc;
+}
+
+// Drop-in for the combined scatter() (scatter.cu:2013-2378): crystal and
+// collimator terms of one rotation (Img[20]), summed into dst (N x V).  The
+// bitmaps are computed (and written to the files) when Phys[8] / Phys[9] are 1,
+// otherwise read from the files, as the reference does.
+int sm_scatter(const float* col, const float* det, const float* img, const float* phy, const float* pe,
+               const char* fname_geo_crystal, const char* fname_geo_collimator, float* dst, int cuda_id) {
+    clear_error();
+    if (!col || !det || !img || !phy || !pe || !dst) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(cuda_id);
+    if (rc < 0) return rc;
+    const int rot = (int)std::floor(img[20] + 0.001f);
+    auto* P = new SmScatterPlan();
+    std::unique_ptr<SmScatterPlan> guard(P);
+    if ((rc = build_geom(col, det, img, phy, cuda_id, rot, P->G)) < 0) return rc;
+    const bool calc_c = (int)std::floor(phy[8]) == 1, calc_h = (int)std::floor(phy[9] + 0.001f) == 1;
+    if ((rc = make_bitmap(P->G, 0, calc_c, fname_geo_crystal, true, P->bits_crystal, nullptr)) < 0) return rc;
+    if ((rc = make_bitmap(P->G, 1, calc_h, fname_geo_collimator, true, P->bits_collimator, nullptr)) < 0) return rc;
+    const long long V = P->G.base.V, N = P->G.T.n_det;
+    DeviceBuf<float> dpe, dc, dh;
+    if ((rc = dpe.alloc((size_t)(N * V))) < 0 || (rc = dc.alloc((size_t)(N * V))) < 0 ||
+        (rc = dh.alloc((size_t)(N * V))) < 0)
+        return rc;
+    HIP_TRY(hipMemcpy(dpe.ptr, pe, sizeof(float) * N * V, hipMemcpyHostToDevice));
+    if ((rc = run_scatter(P->G, 0, P->bits_crystal.ptr, dpe.ptr, 0, (int)N, dc.ptr, V, nullptr)) < 0) return rc;
+    if ((rc = run_scatter(P->G, 1, P->bits_collimator.ptr, nullptr, 0, (int)N, dh.ptr, V, nullptr)) < 0) return rc;
+    std::vector<float> a((size_t)(N * V)), b((size_t)(N * V));
+    HIP_TRY(hipMemcpy(a.data(), dc.ptr, sizeof(float) * N * V, hipMemcpyDeviceToHost));
+    HIP_TRY(hipMemcpy(b.data(), dh.ptr, sizeof(float) * N * V, hipMemcpyDeviceToHost));
+    for (long long i = 0; i < N * V; ++i) dst[i] = b[i] + a[i];  // collimator + crystal (scatter.cu:2354-2357)
+    return (int)V;
+}
+
+// Drop-in for ScatterGen_Collimator's scatter() (ScatterGen_Collimator/scatter.cu:1511-1747).
+int sm_scatter_collimator_only(const float* col, const float* det, const float* img, const float* phy,
+                               const float* pe, const char* fname_geo, float* dst, int cuda_id) {
+    clear_error();
+    (void)pe;
+    if (!col || !det || !img || !phy || !dst) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(cuda_id);
+    if (rc < 0) return rc;
+    const int rot = (int)std::floor(img[20] + 0.001f);
+    auto* P = new SmScatterPlan();
+    std::unique_ptr<SmScatterPlan> guard(P);
+    if ((rc = build_geom(col, det, img, phy, cuda_id, rot, P->G)) < 0) return rc;
+    const bool calc_h = (int)std::floor(phy[9]) == 1;
+    if ((rc = make_bitmap(P->G, 1, calc_h, fname_geo, true, P->bits_collimator, nullptr)) < 0) return rc;
+    const long long V = P->G.base.V, N = P->G.T.n_det;
+    DeviceBuf<float> dh;
+    if ((rc = dh.alloc((size_t)(N * V))) < 0) return rc;
+    if ((rc = run_scatter(P->G, 2, P->bits_collimator.ptr, nullptr, 0, (int)N, dh.ptr, V, nullptr)) < 0) return rc;
+    HIP_TRY(hipMemcpy(dst, dh.ptr, sizeof(float) * N * V, hipMemcpyDeviceToHost));
+    return (int)V;
+}
+
+// Drop-in for ScatterGen_Crystal's scatter() (ScatterGen_Crystal/scatter.cu:1528-1738).
+int sm_scatter_crystal_only(const float* det, const float* img, const float* phy, const float* pe,
+                            const char* fname_geo, float* dst, int cuda_id) {
+    clear_error();
+    if (!det || !img || !phy || !pe || !dst) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(cuda_id);
+    if (rc < 0) return rc;
+    const int rot = (int)std::floor(img[20] + 0.001f);
+    float col0[110] = {0};
+    auto* P = new SmScatterPlan();
+    std::unique_ptr<SmScatterPlan> guard(P);
+    if ((rc = build_geom(col0, det, img, phy, cuda_id, rot, P->G)) < 0) return rc;
+    const bool calc_c = (int)std::floor(phy[8]) == 1;
+    if ((rc = make_bitmap(P->G, 0, calc_c, fname_geo, true, P->bits_crystal, nullptr)) < 0) return rc;
+    const long long V = P->G.base.V, N = P->G.T.n_det;
+    DeviceBuf<float> dpe, dc;
+    if ((rc = dpe.alloc((size_t)(N * V))) < 0 || (rc = dc.alloc((size_t)(N * V))) < 0) return rc;
+    HIP_TRY(hipMemcpy(dpe.ptr, pe, sizeof(float) * N * V, hipMemcpyHostToDevice));
+    if ((rc = run_scatter(P->G, 0, P->bits_crystal.ptr, dpe.ptr, 0, (int)N, dc.ptr, V, nullptr)) < 0) return rc;
+    HIP_TRY(hipMemcpy(dst, dc.ptr, sizeof(float) * N * V, hipMemcpyDeviceToHost));
+    return (int)V;
+}
+
+}  // extern "C"
+
+// C++-linkage drop-ins with the reference's exact signatures.
+int scatter(float* parameter_Collimator, float* parameter_Detector, float* parameter_Image, float* parameter_Physics,
+            float* PE_SysMat, const char* FnameGeoCrystal, const char* FnameGeoCollimator, float* dst, int cuda_id) {
+    int rc = sm_scatter(parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics, PE_SysMat,
+                        FnameGeoCrystal, FnameGeoCollimator, dst, cuda_id);
+    if (rc < 0) std::fprintf(stderr, "scatter: %s\n", sm_last_error());
+    return rc;
+}
+
+int scatter(float* parameter_Collimator, float* parameter_Detector, float* parameter_Image, float* parameter_Physics,
+            float* PE_SysMat, const char* FnameGeo, float* dst, int cuda_id) {
+    int rc = sm_scatter_collimator_only(parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics,
+                                        PE_SysMat, FnameGeo, dst, cuda_id);
+    if (rc < 0) std::fprintf(stderr, "scatter: %s\n", sm_last_error());
+    return rc;
+}
+
+int scatter(float* parameter_Detector, float* parameter_Image, float* parameter_Physics, float* PE_SysMat,
+            const char* FnameGeo, float* dst, int cuda_id) {
+    int rc = sm_scatter_crystal_only(parameter_Detector, parameter_Image, parameter_Physics, PE_SysMat, FnameGeo, dst,
+                                     cuda_id);
+    if (rc < 0) std::fprintf(stderr, "scatter: %s\n", sm_last_error());
+    return rc;
+}

@@ -1,0 +1,79 @@
+// Primary-Compton scatter system matrices: host/device data layout.
+//
+// Semantics: crystalScatterSysMatCuda (ScatterGen_RayTracing_CircularHole/scatter.cu:1077-1437)
+// and collimatorScatterSysMatCuda (scatter.cu:1544-1936; ScatterGen_Collimator
+// variant scatter.cu:1056-1436).  Element (detector r, voxel v) = sum over
+// scatter sites s (crystals k != r, or layer-0 holes) of
+//   w_s(v) * Range_phi/2pi * KN(theta_min..theta_max)/KN(0..pi) * P_window(E')
+//   * exp(-att(E')) * (1 - exp(-L_abs mu(E'))) * mu_pe(E')/mu(E').
+// Everything that does not depend on the voxel (site position in r's frame,
+// |Q - S|, the absorption chord, the attenuator chord sums behind the
+// GeometryRelationShip bitmaps) is computed once per (r, s) pair by
+// pair_prep; the per-voxel kernel then does O(1) work per (r, v, s) triple:
+// the reference's O(N) bitmap scan and O(n_theta) Riemann sum per triple
+// become a 2-term attenuation formula and an Euler-Maclaurin closed form.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "tables.hpp"
+
+namespace sysmat {
+
+// Per detecting crystal r (FP64 where the value feeds a branch).
+struct alignas(16) ScRow {
+    double cx, cy, cz;        // crystal centre (sub-point, divide 1x1x1), FOV2C added in FP64
+    double crn, srn;          // cos/sin(-rot): world -> crystal frame
+    double hx, hy, hz;        // half extents
+    double lo, hi;            // energy window (same-window flag or +-res/2 around E0)
+    double res;               // energy resolution at E0
+    double mu_pe, mu_c;       // photoelectric / Compton attenuation of r at E0
+    double rdet;              // sqrt(w^2 + h^2 + t^2) / 2 (Range_phi sphere)
+    double th_lo, th_hi;      // scatter angles outside which the window test always fails
+    float use;                // 1 when r contributes (mu_t > 0.01, Compton on)
+    float pad0, pad1, pad2;
+};
+
+// Per (r, site) pair, 192 B.
+struct alignas(16) ScPair {
+    double sx, sy, sz;        // site (world)
+    double ssx, ssy, ssz;     // site in r's frame
+    double bx, by, bz;        // unit vector site -> r's sub-point (world)
+    double L;                 // |Q - S|
+    double ib[8];             // 1 / |corner_i - S_self|, corners in (x, y, z) bit order below
+    float A_pe, A_c;          // sum over bitmap attenuators of chord * mu_pe, chord * mu_c (at E0)
+    float l_abs;              // absorption chord through r along S -> Q extended 1000 mm
+    float c_fwd;              // cos(asin(R/L)) + margin: "through" test needed when |cos theta| >= this
+    int site;                 // crystal index (crystal mode) or hole index (collimator mode)
+    float hy1, hy2;           // hole faces (collimator mode, as stored)
+    float pad;
+};
+
+struct ScParams {
+    int X, Y, Z;              // image voxel counts (Img[0..2])
+    double wx, wy, wz, sx, sy, sz;
+    double rc, rs;            // image rotation (Img[20] * Img[7])
+    double e0;                // photopeak energy (Phys[7])
+    double kn_total;          // KN Riemann sum over [0, pi] (reference normalisation)
+    double h;                 // KN step (0.01f)
+    int n_det;
+    int det_begin, det_count; // detector rows of this launch
+    int max_sites;            // pair table stride per row
+    long long row_stride;     // floats between detector rows of dst
+    long long V;              // voxels (PE matrix row length)
+    int mode;                 // 0 crystal scatter, 1 collimator (combined), 2 collimator (ScatterGen_Collimator)
+    double w_col;             // collimator source factor Col[17] / Col[15]
+    double fov2c;
+    int n_tiles;              // voxel tiles of 64
+    int n_rgroups;            // detector groups of 4
+    long long n_blocks;       // tiles * rgroups (before XCD padding)
+};
+
+hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Crystal* cry, const Hole* hol,
+                               const unsigned* bits, ScPair* pairs, int* counts, hipStream_t stream);
+hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pairs, const int* counts,
+                          const float* pe, float* dst, hipStream_t stream);
+hipError_t launch_bitmap_crystal(const Crystal* cry, int n, unsigned* words, hipStream_t stream);
+hipError_t launch_bitmap_collimator(const Hole* hol, int n_holes, const Crystal* cry, int n, unsigned* words,
+                                    hipStream_t stream);
+
+}  // namespace sysmat

@@ -244,3 +244,62 @@ __device__ inline double chord_hole64(const Ray64& r, double xc, double y1, doub
 }
 
 }  // namespace sysmat
+
+namespace sysmat {
+
+// length_box_ray in FP64 with the reference's full branch structure
+// (PESysMatGen.cu:19-151 / scatter.cu:31-323), for per-pair scatter chords.
+__host__ __device__ inline double slab2_64(double ai, double bi, double da, double db, double a1, double a2, double b1,
+                                           double b2) {
+    const double eps = (double)0.001f;
+    const double t = sqrt(da * da + db * db);
+    const double ia = t / da, ib = t / db;
+    double lo, hi, lo2, hi2;
+    if (ia < 0) { lo = (a2 - ai) * ia; hi = (a1 - ai) * ia; } else { hi = (a2 - ai) * ia; lo = (a1 - ai) * ia; }
+    if (ib < 0) { lo2 = (b2 - bi) * ib; hi2 = (b1 - bi) * ib; } else { hi2 = (b2 - bi) * ib; lo2 = (b1 - bi) * ib; }
+    if (lo > hi2 || lo2 > hi) return 0.0;
+    if (lo2 > lo) lo = lo2;
+    if (hi2 < hi) hi = hi2;
+    if ((hi - lo) < eps) return 0.0;
+    if (lo >= t || hi >= t) return 0.0;
+    if (lo <= eps || hi <= eps) return 0.0;
+    return hi - lo;
+}
+
+__host__ __device__ inline double chord_box64(double xi, double yi, double zi, double xo, double yo, double zo,
+                                              double x1, double y1, double z1, double x2, double y2, double z2) {
+    const double eps = (double)0.001f;
+    const double dx = xo - xi, dy = yo - yi, dz = zo - zi;
+    const bool px = fabs(dx) < eps, py = fabs(dy) < eps, pz = fabs(dz) < eps;
+    if (px && py && pz) return 0.0;
+    if (px && py)
+        return (xi >= x1 && xi <= x2 && yi >= y1 && yi <= y2 && ((zi <= z1 && zo >= z2) || (zo <= z1 && zi >= z2)))
+                   ? fabs(z2 - z1) : 0.0;
+    if (pz && py)
+        return (zi >= z1 && zi <= z2 && yi >= y1 && yi <= y2 && ((xi <= x1 && xo >= x2) || (xo <= x1 && xi >= x2)))
+                   ? fabs(x2 - x1) : 0.0;
+    if (px && pz)
+        return (xi >= x1 && xi <= x2 && zi >= z1 && zi <= z2 && ((yi <= y1 && yo >= y2) || (yo <= y1 && yi >= y2)))
+                   ? fabs(y2 - y1) : 0.0;
+    if (px) return (xi >= x2 || xi <= x1) ? 0.0 : slab2_64(yi, zi, dy, dz, y1, y2, z1, z2);
+    if (py) return (yi >= y2 || yi <= y1) ? 0.0 : slab2_64(xi, zi, dx, dz, x1, x2, z1, z2);
+    if (pz) return (zi >= z2 || zi <= z1) ? 0.0 : slab2_64(xi, yi, dx, dy, x1, x2, y1, y2);
+    const double t = sqrt(dx * dx + dy * dy + dz * dz);
+    const double ix = t / dx, iy = t / dy, iz = t / dz;
+    double lo, hi, lo2, hi2;
+    if (ix < 0) { lo = (x2 - xi) * ix; hi = (x1 - xi) * ix; } else { hi = (x2 - xi) * ix; lo = (x1 - xi) * ix; }
+    if (iy < 0) { lo2 = (y2 - yi) * iy; hi2 = (y1 - yi) * iy; } else { hi2 = (y2 - yi) * iy; lo2 = (y1 - yi) * iy; }
+    if (lo > hi2 || lo2 > hi) return 0.0;
+    if (lo2 > lo) lo = lo2;
+    if (hi2 < hi) hi = hi2;
+    if (iz < 0) { lo2 = (z2 - zi) * iz; hi2 = (z1 - zi) * iz; } else { hi2 = (z2 - zi) * iz; lo2 = (z1 - zi) * iz; }
+    if (lo > hi2 || lo2 > hi) return 0.0;
+    if (lo2 > lo) lo = lo2;
+    if (hi2 < hi) hi = hi2;
+    if ((hi - lo) < eps) return 0.0;
+    if (lo >= t || hi >= t) return 0.0;
+    if (lo <= eps || hi <= eps) return 0.0;
+    return hi - lo;
+}
+
+}  // namespace sysmat

@@ -54,6 +54,22 @@ SIGNATURES = {
     "sm_pe_plan_sync": (ctypes.c_int, [_VP]),
     "sm_pe_plan_destroy": (ctypes.c_int, [_VP]),
     "sm_pe_plan_subrays": (ctypes.c_longlong, [_VP]),
+    "sm_crystal_bitmap_words": (ctypes.c_longlong, [ctypes.c_int]),
+    "sm_collimator_bitmap_words": (ctypes.c_longlong, [ctypes.c_int, ctypes.c_int]),
+    "sm_bitmap_crystal": (ctypes.c_int, [_VP, _VP, ctypes.c_longlong, ctypes.c_int]),
+    "sm_bitmap_collimator": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_longlong, ctypes.c_int]),
+    "sm_scatter": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_char_p, ctypes.c_char_p, _VP, ctypes.c_int]),
+    "sm_scatter_collimator_only": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_char_p, _VP, ctypes.c_int]),
+    "sm_scatter_crystal_only": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_char_p, _VP, ctypes.c_int]),
+    "sm_scatter_crystal": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP,
+                                          ctypes.c_int]),
+    "sm_scatter_collimator": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _VP, ctypes.c_int]),
+    "sm_scatter_plan_create": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_int,
+                                              ctypes.POINTER(_VP)]),
+    "sm_scatter_plan_crystal": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_scatter_plan_collimator": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_scatter_plan_destroy": (ctypes.c_int, [_VP]),
 }
 
 _lib = None

@@ -1,0 +1,128 @@
+"""Primary-Compton scatter matrices: Python face of the HIP kernels.
+
+Mirrors the reference's ``scatter(...)`` operators
+(ScatterGen_RayTracing_CircularHole/scatter.h:4-5, ScatterGen_Collimator/scatter.h:4-5,
+ScatterGen_Crystal/scatter.h:4): the combined crystal + collimator scatter of
+one rotation (Img[20]) of an N x V PE matrix, with the GeometryRelationShip
+bitmaps computed on the GPU or read from files (Phys[8] / Phys[9]).
+Errors raise ``SysmatError`` (the reference exits with status 0 on a missing
+file, scatter.cu:2145-2157).
+"""
+from __future__ import annotations
+
+import ctypes
+from typing import Optional, Tuple
+
+import numpy as np
+
+from ._lib import check, load, ptr
+from .params import Geometry
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _img(img, rot: Optional[int] = None) -> np.ndarray:
+    out = np.zeros(max(100, len(img)), np.float32)
+    out[:len(img)] = img
+    if rot is not None:
+        out[20] = rot
+    return out
+
+
+def n_holes_layer0(geo: Geometry) -> int:
+    return int(np.floor(geo.col[10] + 1e-6)) if geo.n_layers > 0 else 0
+
+
+def bitmap_crystal(geo: Geometry, device: int = 0) -> np.ndarray:
+    """GeometryRelationShip_Crystal2Crystal words, computed on the GPU."""
+    L = load()
+    w = np.zeros(int(L.sm_crystal_bitmap_words(geo.n_det)), np.uint32)
+    check(L.sm_bitmap_crystal(ptr(_f32(geo.det)), ptr(w), w.size, device))
+    return w
+
+
+def bitmap_collimator(geo: Geometry, device: int = 0) -> np.ndarray:
+    """GeometryRelationShip_Collimator2Crystal words, computed on the GPU."""
+    L = load()
+    w = np.zeros(max(1, int(L.sm_collimator_bitmap_words(n_holes_layer0(geo), geo.n_det))), np.uint32)
+    check(L.sm_bitmap_collimator(ptr(_f32(geo.col)), ptr(_f32(geo.det)), ptr(w), w.size, device))
+    return w
+
+
+def scatter_crystal(geo: Geometry, pe_sysmat: np.ndarray, bits: Optional[np.ndarray] = None, rotation: int = 0,
+                    det_range: Optional[Tuple[int, int]] = None, device: int = 0) -> np.ndarray:
+    """crystalScatterSysMatCuda rows for the N x V PE matrix (host arrays in and out)."""
+    b, e = det_range if det_range is not None else (0, geo.n_det)
+    pe = _f32(pe_sysmat).reshape(geo.n_det, geo.n_vox)
+    dst = np.zeros((e - b, geo.n_vox), np.float32)
+    bw = None if bits is None else np.ascontiguousarray(bits, np.uint32)
+    check(load().sm_scatter_crystal(ptr(_f32(geo.det)), ptr(_img(geo.img)), ptr(_f32(geo.phy)), ptr(pe),
+                                    None if bw is None else ptr(bw), rotation, b, e - b, ptr(dst), device))
+    return dst
+
+
+def scatter_collimator(geo: Geometry, bits: Optional[np.ndarray] = None, rotation: int = 0, variant: int = 0,
+                       det_range: Optional[Tuple[int, int]] = None, device: int = 0) -> np.ndarray:
+    """collimatorScatterSysMatCuda rows (variant 0) or the ScatterGen_Collimator kernel (variant 1)."""
+    b, e = det_range if det_range is not None else (0, geo.n_det)
+    dst = np.zeros((e - b, geo.n_vox), np.float32)
+    bw = None if bits is None else np.ascontiguousarray(bits, np.uint32)
+    check(load().sm_scatter_collimator(ptr(_f32(geo.col)), ptr(_f32(geo.det)), ptr(_img(geo.img)), ptr(_f32(geo.phy)),
+                                       None if bw is None else ptr(bw), rotation, variant, b, e - b, ptr(dst), device))
+    return dst
+
+
+def scatter(parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics, PE_SysMat,
+            FnameGeoCrystal: str, FnameGeoCollimator: str, dst: np.ndarray, cuda_id: int = 0) -> int:
+    """Drop-in for the combined reference scatter(); ``dst`` (N*V float32) is written in place."""
+    det = _f32(parameter_Detector)
+    img = _img(parameter_Image)
+    N = int(np.floor(det[0] + 1e-4))
+    V = int(np.floor(img[0] + 1e-3)) * int(np.floor(img[1] + 1e-3)) * int(np.floor(img[2] + 1e-3))
+    pe = _f32(PE_SysMat)
+    if pe.size < N * V or dst.size < N * V or dst.dtype != np.float32:
+        raise ValueError("PE_SysMat and dst must hold N*V float32 values")
+    return check(load().sm_scatter(ptr(_f32(parameter_Collimator)), ptr(det), ptr(img), ptr(_f32(parameter_Physics)),
+                                   ptr(pe), FnameGeoCrystal.encode(), FnameGeoCollimator.encode(), ptr(dst), cuda_id))
+
+
+class ScatterPlan:
+    """Geometry, bitmaps and tables resident in HBM; row blocks into device tensors."""
+
+    def __init__(self, geo: Geometry, rotation: int = 0, variant: int = 0, with_collimator: bool = True,
+                 crystal_bits: Optional[np.ndarray] = None, collimator_bits: Optional[np.ndarray] = None,
+                 device: int = 0):
+        self._lib = load()
+        self.geo = geo
+        self._keep = [_f32(geo.col), _f32(geo.det), _img(geo.img), _f32(geo.phy)]
+        cb = None if crystal_bits is None else np.ascontiguousarray(crystal_bits, np.uint32)
+        hb = None if collimator_bits is None else np.ascontiguousarray(collimator_bits, np.uint32)
+        h = ctypes.c_void_p()
+        check(self._lib.sm_scatter_plan_create(ptr(self._keep[0]) if with_collimator else None, ptr(self._keep[1]),
+                                               ptr(self._keep[2]), ptr(self._keep[3]), rotation, variant,
+                                               None if cb is None else ptr(cb), None if hb is None else ptr(hb),
+                                               device, ctypes.byref(h)))
+        self._h = h
+
+    def crystal(self, pe_dev, det_begin: int, det_count: int, dst_dev, row_stride: int = 0, stream=None) -> None:
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        check(self._lib.sm_scatter_plan_crystal(self._h, ptr(pe_dev), det_begin, det_count, ptr(dst_dev), row_stride,
+                                                ctypes.c_void_p(s) if s else None))
+
+    def collimator(self, det_begin: int, det_count: int, dst_dev, row_stride: int = 0, stream=None) -> None:
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        check(self._lib.sm_scatter_plan_collimator(self._h, det_begin, det_count, ptr(dst_dev), row_stride,
+                                                   ctypes.c_void_p(s) if s else None))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sm_scatter_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

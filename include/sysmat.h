@@ -85,6 +85,53 @@ int sm_pe_plan_destroy(SmPePlan* plan);
 /* Algorithmic sub-ray count per element of the plan (voxel_sub^3 * sub_x*sub_y*sub_z). */
 long long sm_pe_plan_subrays(const SmPePlan* plan);
 
+/* ---- primary Compton scatter ------------------------------------------- */
+
+/* GeometryRelationShip bitmaps (raw uint32 words, LSB-first, the reference's
+ * file layout): crystal bit ((pair(min,max) of scatter/detector) * N + k),
+ * pair(i,j) = i*N - i(i+1)/2 + j (scatter.cu:1040-1058, builder 1440-1540);
+ * collimator bit (hole*N*N + det*N + k) (scatter.cu:1061-1074, builder 1939-2009). */
+long long sm_crystal_bitmap_words(int n_det);
+long long sm_collimator_bitmap_words(int n_holes_layer0, int n_det);
+int sm_bitmap_crystal(const float* det, unsigned* words, long long n_words, int device);
+int sm_bitmap_collimator(const float* col, const float* det, unsigned* words, long long n_words, int device);
+
+/* Drop-in for the combined scatter() (ScatterGen_RayTracing_CircularHole/
+ * scatter.cu:2013-2378): crystal + collimator scatter of rotation Img[20] for the
+ * N x V PE matrix `pe`; bitmaps computed and written when Phys[8]/[9] == 1, else
+ * read from the files.  dst: N x V floats.  Returns V or an SM_E* code. */
+int sm_scatter(const float* col, const float* det, const float* img, const float* phy, const float* pe,
+               const char* fname_geo_crystal, const char* fname_geo_collimator, float* dst, int cuda_id);
+/* Drop-in for ScatterGen_Collimator's scatter() (ScatterGen_Collimator/scatter.cu:1511-1747). */
+int sm_scatter_collimator_only(const float* col, const float* det, const float* img, const float* phy,
+                               const float* pe, const char* fname_geo, float* dst, int cuda_id);
+/* Drop-in for ScatterGen_Crystal's scatter() (ScatterGen_Crystal/scatter.cu:1528-1738). */
+int sm_scatter_crystal_only(const float* det, const float* img, const float* phy, const float* pe,
+                            const char* fname_geo, float* dst, int cuda_id);
+
+/* Host-memory row blocks: crystal scatter (crystalScatterSysMatCuda, scatter.cu:1077-1437)
+ * and collimator scatter (variant 0: collimatorScatterSysMatCuda scatter.cu:1544-1936,
+ * variant 1: ScatterGen_Collimator/scatter.cu:1056-1436).  Bitmap words may be NULL
+ * (then computed on the GPU).  dst: det_count x V floats. */
+int sm_scatter_crystal(const float* det, const float* img, const float* phy, const float* pe,
+                       const unsigned* crystal_words, int rotation, int det_begin, int det_count, float* dst,
+                       int device);
+int sm_scatter_collimator(const float* col, const float* det, const float* img, const float* phy,
+                          const unsigned* collimator_words, int rotation, int variant, int det_begin, int det_count,
+                          float* dst, int device);
+
+/* Plans: geometry, bitmaps and per-row tables resident in HBM.  col may be NULL
+ * (crystal scatter only).  Launches are synchronous on `stream`. */
+typedef struct SmScatterPlan SmScatterPlan;
+int sm_scatter_plan_create(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                           int collimator_variant, const unsigned* crystal_words, const unsigned* collimator_words,
+                           int device, SmScatterPlan** plan);
+int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
+                            long long row_stride, void* stream);
+int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
+                               long long row_stride, void* stream);
+int sm_scatter_plan_destroy(SmScatterPlan* plan);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,130 @@
+"""GPU parity of the Compton-scatter kernels against the CPU oracle.
+
+Bitmaps (GeometryRelationShip_*): bit-exact against the FP32 restatement of the
+reference builders (scatter.cu:1440-1540, 1939-2009).
+
+Scatter matrices: every element at or above 1e-9 of its row maximum within
+1e-5 relative of the FP64 oracle (crystalScatterSysMatCuda /
+collimatorScatterSysMatCuda restated in double, same bitmaps and PE input);
+rows that are all zero in the oracle are all zero on the GPU.
+"""
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from sysmat import params
+from sysmat import scatter as sc
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def assert_parity(gpu, ref, rel=REL, what=""):
+    gpu = np.asarray(gpu, np.float64)
+    ref = np.asarray(ref, np.float64)
+    assert gpu.shape == ref.shape
+    rowmax = ref.max(axis=1, keepdims=True)
+    zero_rows = rowmax[:, 0] <= 0
+    assert np.all(gpu[zero_rows] == 0), f"{what}: rows zero in the oracle are not zero on the GPU"
+    sig = (ref >= rowmax * 1e-9) & (rowmax > 0)
+    rel_err = np.where(sig, np.abs(gpu - ref) / np.maximum(ref, 1e-300), 0.0)
+    worst = float(rel_err.max()) if rel_err.size else 0.0
+    print(f"[{what}] max rel err = {worst:.3e}, significant = {sig.mean():.3f}, "
+          f"total sum rel = {abs(gpu.sum() - ref.sum()) / max(ref.sum(), 1e-300):.2e}", flush=True)
+    assert worst <= rel, f"{what}: max relative error {worst:.3e} > {rel:.0e}"
+    small = ~sig & ~zero_rows[:, None]
+    if small.any():
+        assert np.all(np.abs(gpu - ref)[small] <= (rowmax * 1e-14).repeat(ref.shape[1], 1)[small])
+
+
+def geo_small(seed=1, side=4, layers=2, image=(4, 4, 4), holes=3, plates=1):
+    return params.tiny(seed=seed, n_det_side=side, n_layers=layers, image=image, holes_side=holes, n_plates=plates)
+
+
+def test_bitmaps_bit_exact(lib):
+    for seed, side, layers in ((1, 4, 2), (2, 5, 3), (3, 6, 2)):
+        g = geo_small(seed, side, layers, holes=3)
+        assert np.array_equal(sc.bitmap_crystal(g), oracle.bitmap_crystal(g)), f"crystal bitmap seed {seed}"
+        assert np.array_equal(sc.bitmap_collimator(g), oracle.bitmap_collimator(g)), f"collimator bitmap seed {seed}"
+
+
+def test_crystal_scatter_parity(lib):
+    for seed in (1, 2):
+        g = geo_small(seed)
+        pe = oracle.pe(g, precision="f64")
+        bits = oracle.bitmap_crystal(g)
+        gpu = sc.scatter_crystal(g, pe, bits)
+        assert_parity(gpu, oracle.scatter_crystal(g, pe, bits, precision="f64"), what=f"crystal seed {seed}")
+        f32 = oracle.scatter_crystal(g, pe, bits, precision="f32")
+        assert_parity(gpu, f32, rel=1e-4, what=f"crystal vs f32 seed {seed}")
+
+
+@pytest.mark.parametrize("variant", [0, 1])
+def test_collimator_scatter_parity(lib, variant):
+    for seed in (1, 2):
+        g = geo_small(seed)
+        bits = oracle.bitmap_collimator(g)
+        gpu = sc.scatter_collimator(g, bits, variant=variant)
+        ref = oracle.scatter_collimator(g, bits, precision="f64", variant=variant)
+        assert_parity(gpu, ref, what=f"collimator v{variant} seed {seed}")
+
+
+def test_rotation_and_window_variants(lib):
+    g = geo_small(4, side=3, layers=2, image=(3, 4, 3))
+    g.img[7] = 0.3
+    g.phy[4] = 0  # per-crystal window (1 +- res/2) E0
+    g.crystals[:, 10] = np.linspace(-0.2, 0.3, g.n_det)  # rotated crystals
+    pe = oracle.pe(g, rot=1, precision="f64", sub=(2, 2, 2))
+    cb, hb = oracle.bitmap_crystal(g), oracle.bitmap_collimator(g)
+    assert_parity(sc.scatter_crystal(g, pe, cb, rotation=1),
+                  oracle.scatter_crystal(g, pe, cb, precision="f64", rot=1), what="crystal rot")
+    assert_parity(sc.scatter_collimator(g, hb, rotation=1),
+                  oracle.scatter_collimator(g, hb, precision="f64", rot=1), what="collimator rot")
+
+
+def test_config3_rows(lib):
+    """C3 detector (2 x 32 x 32 crystals, 6x6 holes) on a reduced 8^3 image, GPU bitmaps."""
+    g = params.config3()
+    g.img[0:3] = 8
+    g.img[3:6] = 4.0
+    rows = (1024 + 16 * 32 + 14, 1024 + 16 * 32 + 18)  # back layer, centre
+    pe = oracle.pe(g, precision="f64", sub=(2, 2, 2))
+    cb = sc.bitmap_crystal(g)
+    hb = sc.bitmap_collimator(g)
+    gpu_c = sc.scatter_crystal(g, pe, cb, det_range=rows)
+    assert_parity(gpu_c, oracle.scatter_crystal(g, pe, cb, det_range=rows, precision="f64"), what="c3 crystal")
+    gpu_h = sc.scatter_collimator(g, hb, det_range=rows)
+    assert_parity(gpu_h, oracle.scatter_collimator(g, hb, det_range=rows, precision="f64"), what="c3 collimator")
+
+
+def test_dropin_scatter_files(lib, tmp_path):
+    """scatter() with Phys[8]/[9] = 1 writes the bitmap files; with 0 it reads them back
+    (scatter.cu:2116-2164, 2246-2296); dst = collimator + crystal."""
+    g = geo_small(5)
+    pe = oracle.pe(g, precision="f64")
+    fc, fh = str(tmp_path / "GeometryRelationShip_Crystal2Crystal"), str(tmp_path / "GeometryRelationShip_Collimator2Crystal")
+    phy = g.phy.copy()
+    phy[8] = phy[9] = 1
+    dst = np.zeros(g.n_det * g.n_vox, np.float32)
+    assert sc.scatter(g.col, g.det, g.img, phy, pe, fc, fh, dst) == g.n_vox
+    assert np.array_equal(np.fromfile(fc, np.uint32), oracle.bitmap_crystal(g))
+    assert np.array_equal(np.fromfile(fh, np.uint32), oracle.bitmap_collimator(g))
+    phy[8] = phy[9] = 0
+    dst2 = np.zeros_like(dst)
+    sc.scatter(g.col, g.det, g.img, phy, pe, fc, fh, dst2)
+    assert np.array_equal(dst, dst2)
+    both = sc.scatter_collimator(g) + sc.scatter_crystal(g, pe)
+    assert np.array_equal(dst.reshape(g.n_det, -1), both)
+
+
+def test_missing_bitmap_file_is_an_error(lib, tmp_path):
+    g = geo_small(6)
+    pe = np.zeros((g.n_det, g.n_vox), np.float32)
+    phy = g.phy.copy()
+    phy[8] = phy[9] = 0
+    dst = np.zeros(g.n_det * g.n_vox, np.float32)
+    with pytest.raises(Exception):
+        sc.scatter(g.col, g.det, g.img, phy, pe, str(tmp_path / "nope"), str(tmp_path / "nope2"), dst)
