@@ -70,6 +70,17 @@ SIGNATURES = {
     "sm_scatter_plan_crystal": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
     "sm_scatter_plan_collimator": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
     "sm_scatter_plan_destroy": (ctypes.c_int, [_VP]),
+    "sm_csr_count_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,
+                                           _VP, _VP, ctypes.POINTER(ctypes.c_longlong), _VP]),
+    "sm_csr_fill_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,
+                                          _VP, _VP, _VP, _VP, _VP]),
+    "sm_pe_compute_csr": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), ctypes.c_double,
+                                         ctypes.POINTER(_VP)]),
+    "sm_csr_info": (ctypes.c_int, [_VP, ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong),
+                                   ctypes.POINTER(ctypes.c_longlong)]),
+    "sm_csr_copy": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
+    "sm_csr_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.c_int]),
+    "sm_csr_destroy": (ctypes.c_int, [_VP]),
 }
 
 _lib = None

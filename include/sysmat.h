@@ -9,8 +9,8 @@
  *                             scatter.cu:2013-2378
  *   scatter(...) collimator   ScatterGen_Collimator/scatter.h:4-5, scatter.cu:1511-1747
  *   scatter(...) crystal      ScatterGen_Crystal/scatter.h:4, scatter.cu:1528-1738
- * The C++-linkage originals are also exported (see sysmat_ref.h) so the
- * reference driver sources link against this library unchanged.
+ * The C++-linkage originals are also exported, declared in sysmat_ref.h, so
+ * the reference driver sources link against this library unchanged.
  *
  * Parameter arrays are the reference's raw float32 Params_*.dat contents
  * (layout: PEGen_RayTracing_CircularHole/ReadMe.txt).  Counts are read from
@@ -131,6 +131,37 @@ int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_be
 int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
                                long long row_stride, void* stream);
 int sm_scatter_plan_destroy(SmScatterPlan* plan);
+
+/* ---- sparse output stage (no counterpart in the reference, whose only output
+ * is the dense writer PEGen_CircularHole.cpp:201-216; the dense layout stays the
+ * drop-in default).  An element v of a row is kept when v > 0 and
+ * (double)v >= rel * (double)max(0, row max).  Columns ascend within a row. */
+
+/* Row maxima (row_max_dev: rows floats) and row pointers (row_ptr_dev: rows+1
+ * int64, exclusive prefix of the kept counts) of `rows` dense device rows of
+ * `cols` floats, row_stride floats apart (0 = cols).  Synchronises `stream` and
+ * returns the total kept count in *nnz. */
+int sm_csr_count_device(const float* dense_dev, int rows, long long cols, long long row_stride, double rel,
+                        float* row_max_dev, long long* row_ptr_dev, long long* nnz, void* stream);
+/* Fill column indices (int32) and values (float32), nnz entries each, from the
+ * outputs of sm_csr_count_device on the same rows.  Asynchronous on `stream`. */
+int sm_csr_fill_device(const float* dense_dev, int rows, long long cols, long long row_stride, double rel,
+                       const float* row_max_dev, const long long* row_ptr_dev, int* col_dev, float* val_dev,
+                       void* stream);
+
+/* Host-side CSR result of PE rows [det_begin, det_begin+det_count) (options as
+ * sm_pe_compute): computed in dense device blocks, compacted on the GPU. */
+typedef struct SmCsr SmCsr;
+int sm_pe_compute_csr(const float* col, const float* det, const float* img, const SmPeOptions* o, double rel,
+                      SmCsr** out);
+int sm_csr_info(const SmCsr* c, long long* rows, long long* cols, long long* nnz);
+/* Copy out row_ptr (rows+1 int64), col_idx (nnz int32), val (nnz float32); NULL skips one. */
+int sm_csr_copy(const SmCsr* c, long long* row_ptr, int* col_idx, float* val);
+/* Write parts stacked row-wise to a .csr file: 64-byte header ("SMCSR\0\0\1",
+ * int64 rows, int64 cols, int64 nnz, float64 rel, 24 zero bytes), then
+ * row_ptr[rows+1] int64, col[nnz] int32, val[nnz] float32, little endian. */
+int sm_csr_write(const char* path, const SmCsr* const* parts, int n_parts);
+int sm_csr_destroy(SmCsr* c);
 
 #ifdef __cplusplus
 }

@@ -223,3 +223,21 @@ def pe_sensitive(geo, ref: np.ndarray, jitter: float = 2e-5, trials: int = 3, to
         alt = pe(geo, precision="f64", jitter=jitter, jseed=t, **kw).astype(np.float64)
         flag |= np.abs(alt - ref) > tol * np.abs(ref)
     return flag
+
+
+def csr(dense: np.ndarray, rel: float = 1e-9):
+    """Row-wise CSR of a dense matrix, keeping v > 0 with float64(v) >= rel * float64(max(0, row max)).
+
+    Returns (row_ptr int64, col int32, val float32).  The reference writes dense
+    rows only (PEGen_CircularHole.cpp:201-216); this threshold rule is the sparse
+    output stage's definition, pinned by a brute-force loop in the tests."""
+    d = np.ascontiguousarray(dense, np.float32)
+    rmax = np.maximum(np.nan_to_num(d, nan=0.0).max(axis=1, initial=0.0), 0.0) if d.shape[1] else np.zeros(d.shape[0])
+    thr = rel * rmax.astype(np.float64)
+    with np.errstate(invalid="ignore"):
+        keep = (d > 0) & (d.astype(np.float64) >= thr[:, None])
+    counts = keep.sum(axis=1)
+    row_ptr = np.zeros(d.shape[0] + 1, np.int64)
+    np.cumsum(counts, out=row_ptr[1:])
+    rows, cols = np.nonzero(keep)
+    return row_ptr, cols.astype(np.int32), d[rows, cols]

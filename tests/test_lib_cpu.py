@@ -23,8 +23,11 @@ def test_library_exports_every_declared_symbol():
     for n in names:
         assert hasattr(L, n), f"{n} declared in include/sysmat.h but not exported"
         assert n in _lib.SIGNATURES, f"{n} has no ctypes signature"
-    # and the C++-linkage reference drop-in (PESysMatGen.h:4-9)
-    assert hasattr(L, "_Z11PESysMatGenPfS_S_S_i")
+    # and the C++-linkage reference drop-ins declared in include/sysmat_ref.h:
+    # PESysMatGen.h:4-9 and the three scatter.h variants
+    for mangled in ("_Z11PESysMatGenPfS_S_S_i", "_Z7scatterPfS_S_S_S_PKcS1_S_i", "_Z7scatterPfS_S_S_S_PKcS_i",
+                    "_Z7scatterPfS_S_S_PKcS_i"):
+        assert hasattr(L, mangled), f"{mangled} not exported"
 
 
 def test_version_and_errors_without_gpu():
