@@ -81,6 +81,16 @@ SIGNATURES = {
     "sm_csr_copy": (ctypes.c_int, [_VP, _VP, _VP, _VP]),
     "sm_csr_write": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(_VP), ctypes.c_int]),
     "sm_csr_destroy": (ctypes.c_int, [_VP]),
+    "sm_mlp_param_count": (ctypes.c_longlong, []),
+    "sm_mlp_packed_bytes": (ctypes.c_longlong, []),
+    "sm_mlp_workspace_bytes": (ctypes.c_longlong, [ctypes.c_int]),
+    "sm_mlp_init": (ctypes.c_int, [_VP, ctypes.c_ulonglong, _VP]),
+    "sm_mlp_forward": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
+    "sm_mlp_backward": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
+    "sm_mlp_train_step": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
+                                         ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
+    "sm_mlp_pack": (ctypes.c_int, [_VP, _VP, _VP]),
+    "sm_mlp_infer": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
 }
 
 _lib = None

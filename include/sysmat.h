@@ -163,6 +163,35 @@ int sm_csr_copy(const SmCsr* c, long long* row_ptr, int* col_idx, float* val);
 int sm_csr_write(const char* path, const SmCsr* const* parts, int n_parts);
 int sm_csr_destroy(SmCsr* c);
 
+/* ---- scatter-surrogate MLP (no counterpart in the reference) ------------
+ * 12 -> 256 -> 256 -> 256 -> 256 -> 1, SiLU; bf16 operands on the matrix cores,
+ * FP32 accumulation.  Parameters are sm_mlp_param_count() FP32 values in device
+ * memory (layout: W1[256][32] (inputs 12..31 zero), b1, W2[256][256], b2, W3,
+ * b3, W4, b4, w5[256], b5).  Rows of x are 12 FP32 features.  All calls are
+ * asynchronous on `stream`. */
+long long sm_mlp_param_count(void);
+long long sm_mlp_packed_bytes(void);
+/* Device workspace for a training batch (batch: positive multiple of 64). */
+long long sm_mlp_workspace_bytes(int batch);
+/* Uniform +-sqrt(6/fan_in) weights, zero biases, from a counter-based RNG. */
+int sm_mlp_init(float* params_dev, unsigned long long seed, void* stream);
+/* Training-path forward of n <= batch rows: y_dev[n]; keeps activations in ws. */
+int sm_mlp_forward(const float* params_dev, const float* x_dev, int n, int batch, void* ws, long long ws_size,
+                   float* y_dev, void* stream);
+/* Gradient of sum(dy * y) for the rows of the last sm_mlp_forward on ws. */
+int sm_mlp_backward(const float* params_dev, const float* dy_dev, int n, int batch, void* ws, long long ws_size,
+                    float* grad_dev, void* stream);
+/* One Adam step (beta 0.9/0.999, eps 1e-8, step counts from 1) on the loss
+ * sum_i w_i (y_i - t_i)^2 (w_dev NULL: w_i = 1/n); loss_dev (nullable) gets it. */
+int sm_mlp_train_step(float* params_dev, float* adam_m, float* adam_v, const float* x_dev, const float* t_dev,
+                      const float* w_dev, int n, int batch, float lr, int step, void* ws, long long ws_size,
+                      float* loss_dev, void* stream);
+/* Repack parameters for inference (sm_mlp_packed_bytes() bytes per member). */
+int sm_mlp_pack(const float* params_dev, void* packed_dev, void* stream);
+/* Register-resident inference of n rows for n_members consecutive packed
+ * members: y_dev[m*n + i]. */
+int sm_mlp_infer(const void* packed_dev, int n_members, const float* x_dev, long long n, float* y_dev, void* stream);
+
 #ifdef __cplusplus
 }
 #endif
