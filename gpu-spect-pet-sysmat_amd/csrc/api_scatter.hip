@@ -14,11 +14,13 @@
 #include <cstring>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/sysmat.h"
 #include "common.hpp"
 #include "scatter.hpp"
+#include "surrogate.hpp"
 
 namespace sysmat {
 
@@ -181,6 +183,63 @@ static int run_scatter(ScatterGeom& G, int mode, const unsigned* bits_dev, const
     return SM_OK;
 }
 
+// Exact scatter of an explicit detector-row list.  Output row of rows[i] is
+// rows[i] - det_begin (dst_by_row) or i.  With `items` ((index into rows, voxel
+// tile) pairs) only those 64-voxel tiles are computed.  Rows are processed in
+// chunks whose pair tables stay under the pair budget.
+static int run_scatter_list(ScatterGeom& G, int mode, const unsigned* bits, const float* pe_dev,
+                            const std::vector<int>& rows, int det_begin, bool dst_by_row,
+                            const std::vector<int2>* items, float* dst_dev, long long row_stride, hipStream_t s) {
+    if (rows.empty()) return SM_OK;
+    ScParams p = G.base;
+    p.mode = mode;
+    p.max_sites = (mode == 0) ? G.T.n_det : std::max(1, G.n_holes0);
+    p.row_stride = row_stride;
+    p.det_begin = det_begin;
+    p.dst_by_row = dst_by_row ? 1 : 0;
+    p.n_tiles = (int)((p.V + 63) / 64);
+    const long long pair_budget = 2048ll << 20;
+    const int chunk = (int)std::max(1ll, std::min<long long>((long long)rows.size(),
+                                                             pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
+    DeviceBuf<ScPair> pairs;
+    DeviceBuf<int> counts, drows;
+    DeviceBuf<int2> ditems;
+    int rc;
+    if ((rc = pairs.alloc((size_t)chunk * p.max_sites)) < 0) return rc;
+    if ((rc = counts.alloc(chunk)) < 0) return rc;
+    if ((rc = drows.alloc(chunk)) < 0) return rc;
+    std::vector<int2> local;
+    for (size_t c0 = 0; c0 < rows.size(); c0 += chunk) {
+        const int cnt = (int)std::min<size_t>(chunk, rows.size() - c0);
+        HIP_TRY(hipMemcpyAsync(drows.ptr, rows.data() + c0, sizeof(int) * cnt, hipMemcpyHostToDevice, s));
+        p.row_list = drows.ptr;
+        p.det_count = cnt;
+        if (items) {
+            local.clear();
+            for (const int2& it : *items)
+                if (it.x >= (int)c0 && it.x < (int)c0 + cnt) local.push_back(make_int2(it.x - (int)c0, it.y));
+            if (local.empty()) continue;
+        }
+        HIP_TRY(launch_scatter_prep(p, G.rows.ptr, G.cry.ptr, G.hol.ptr, bits, pairs.ptr, counts.ptr, s));
+        float* out = dst_by_row ? dst_dev : dst_dev + (long long)c0 * row_stride;
+        if (items) {
+            if ((rc = ditems.alloc(local.size())) < 0) return rc;
+            HIP_TRY(hipMemcpyAsync(ditems.ptr, local.data(), sizeof(int2) * local.size(), hipMemcpyHostToDevice, s));
+            p.items = ditems.ptr;
+            p.n_items = (long long)local.size();
+        } else {
+            p.items = nullptr;
+            p.n_items = 0;
+            p.n_rgroups = (cnt + 3) / 4;
+            p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
+        }
+        HIP_TRY(launch_scatter(p, G.rows.ptr, pairs.ptr, counts.ptr, pe_dev, out, s));
+        HIP_TRY(hipStreamSynchronize(s));  // host vectors / buffers are reused by the next chunk
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    return SM_OK;
+}
+
 static int read_words(const char* fname, std::vector<unsigned>& w) {
     FILE* f = std::fopen(fname, "rb");
     if (!f) return set_error(SM_E_IO, std::string("cannot open ") + fname);
@@ -233,8 +292,9 @@ using namespace sysmat;
 
 struct SmScatterPlan {
     ScatterGeom G;
-    DeviceBuf<unsigned> bits_crystal, bits_collimator;
+    DeviceBuf<unsigned> bits_crystal, bits_collimator;  // empty: criterion evaluated inline
     int variant = 0;
+    bool has_collimator = false;
 };
 
 extern "C" {
@@ -280,9 +340,9 @@ int sm_bitmap_collimator(const float* col, const float* det, unsigned* words, lo
     return SM_OK;
 }
 
-int sm_scatter_plan_create(const float* col, const float* det, const float* img, const float* phy, int rotation,
-                           int collimator_variant, const unsigned* crystal_words_host,
-                           const unsigned* collimator_words_host, int device, SmScatterPlan** plan) {
+int sm_scatter_plan_create_ex(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                              int collimator_variant, int flags, const unsigned* crystal_words_host,
+                              const unsigned* collimator_words_host, int device, SmScatterPlan** plan) {
     clear_error();
     if (!det || !img || !phy || !plan) return set_error(SM_E_ARG, "null argument");
     int rc = use_device(device);
@@ -291,7 +351,7 @@ int sm_scatter_plan_create(const float* col, const float* det, const float* img,
     auto* P = new SmScatterPlan();
     P->variant = collimator_variant;
     rc = build_geom(col ? col : col0, det, img, phy, device, rotation, P->G);
-    if (rc >= 0) {
+    if (rc >= 0 && !(flags & SM_PLAN_NO_CRYSTAL_BITMAP)) {
         const int n = P->G.T.n_det;
         if (crystal_words_host) {
             const long long nw = crystal_words(n);
@@ -305,8 +365,11 @@ int sm_scatter_plan_create(const float* col, const float* det, const float* img,
         }
     }
     if (rc >= 0 && col) {
+        P->has_collimator = true;
         const long long nw = collimator_words(P->G.n_holes0, P->G.T.n_det);
-        if (collimator_words_host) {
+        if (flags & SM_PLAN_NO_COLLIMATOR_BITMAP) {
+            // criterion evaluated inline by the pair preparation
+        } else if (collimator_words_host) {
             if ((rc = P->bits_collimator.alloc(std::max(1ll, nw))) >= 0 && nw > 0) {
                 hipError_t e = hipMemcpy(P->bits_collimator.ptr, collimator_words_host, sizeof(unsigned) * nw,
                                          hipMemcpyHostToDevice);
@@ -324,6 +387,13 @@ int sm_scatter_plan_create(const float* col, const float* det, const float* img,
     return SM_OK;
 }
 
+int sm_scatter_plan_create(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                           int collimator_variant, const unsigned* crystal_words_host,
+                           const unsigned* collimator_words_host, int device, SmScatterPlan** plan) {
+    return sm_scatter_plan_create_ex(col, det, img, phy, rotation, collimator_variant, 0, crystal_words_host,
+                                     collimator_words_host, device, plan);
+}
+
 int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
                             long long row_stride, void* stream) {
     clear_error();
@@ -337,11 +407,25 @@ int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_be
                        row_stride > 0 ? row_stride : plan->G.base.V, (hipStream_t)stream);
 }
 
+int sm_scatter_plan_crystal_rows(SmScatterPlan* plan, const float* pe_dev, const int* rows, int n_rows,
+                                 float* dst_dev, long long row_stride, void* stream) {
+    clear_error();
+    if (!plan || !pe_dev || !rows || !dst_dev || n_rows <= 0) return set_error(SM_E_ARG, "bad argument");
+    const int n = plan->G.T.n_det;
+    for (int i = 0; i < n_rows; ++i)
+        if (rows[i] < 0 || rows[i] >= n) return set_error(SM_E_ARG, "row outside the detector table");
+    int rc = use_device(plan->G.device);
+    if (rc < 0) return rc;
+    std::vector<int> list(rows, rows + n_rows);
+    return run_scatter_list(plan->G, 0, plan->bits_crystal.ptr, pe_dev, list, 0, false, nullptr, dst_dev,
+                            row_stride > 0 ? row_stride : plan->G.base.V, (hipStream_t)stream);
+}
+
 int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
                                long long row_stride, void* stream) {
     clear_error();
     if (!plan || !dst_dev) return set_error(SM_E_ARG, "null argument");
-    if (!plan->bits_collimator.ptr) return set_error(SM_E_ARG, "plan created without a collimator");
+    if (!plan->has_collimator) return set_error(SM_E_ARG, "plan created without a collimator");
     const int n = plan->G.T.n_det;
     if (det_begin < 0 || det_count <= 0 || det_begin + det_count > n)
         return set_error(SM_E_ARG, "detector range outside the detector table");
@@ -507,3 +591,308 @@ int scatter(float* parameter_Detector, float* parameter_Image, float* parameter_
     if (rc < 0) std::fprintf(stderr, "scatter: %s\n", sm_last_error());
     return rc;
 }
+
+// ---------------------------------------------------------------------------
+// Scatter surrogate (crystal scatter): exact near field + learned far field
+// ---------------------------------------------------------------------------
+
+struct SmSurrogate {
+    ScatterGeom G;
+    DeviceBuf<unsigned> bits;  // crystal bitmap (empty: criterion inline)
+    DeviceBuf<ScPair> keep;    // [N][n_keep] ranked sites
+    DeviceBuf<int> keep_cnt;   // [N]
+    DeviceBuf<SgRow> srow;     // [N]
+    SgParams q{};
+    int n_near = 0, n_ring = 0;
+};
+
+namespace {
+
+// the n_cand crystals nearest to each crystal (centre distance, index order on ties)
+void nearest_candidates(const Tables& T, int n_cand, std::vector<int>& out) {
+    const int n = T.n_det;
+    out.assign((size_t)n * n_cand, -1);
+    auto work = [&](int r0, int r1) {
+        std::vector<std::pair<double, int>> d;
+        d.reserve(n);
+        for (int r = r0; r < r1; ++r) {
+            d.clear();
+            const Crystal& R = T.crystals[r];
+            for (int k = 0; k < n; ++k) {
+                if (k == r) continue;
+                const Crystal& K = T.crystals[k];
+                const double dx = (double)K.cx - R.cx, dy = (double)K.y_raw - R.y_raw, dz = (double)K.cz - R.cz;
+                d.emplace_back(dx * dx + dy * dy + dz * dz, k);
+            }
+            const int m = std::min<int>(n_cand, (int)d.size());
+            std::partial_sort(d.begin(), d.begin() + m, d.end());
+            for (int i = 0; i < m; ++i) out[(size_t)r * n_cand + i] = d[i].second;
+        }
+    };
+    const int nt = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> th;
+    const int per = (n + nt - 1) / nt;
+    for (int t = 0; t < nt; ++t) {
+        const int a = t * per, b = std::min(n, a + per);
+        if (a < b) th.emplace_back(work, a, b);
+    }
+    for (auto& x : th) x.join();
+}
+
+int check_rows(const SmSurrogate* S, int det_begin, int det_count) {
+    if (det_begin < 0 || det_count <= 0 || det_begin + det_count > S->G.T.n_det)
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    return SM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sm_surrogate_create(const float* det, const float* img, const float* phy, int rotation,
+                        const unsigned* crystal_words_host, int use_bitmap, int n_near, int n_ring, int device,
+                        SmSurrogate** out) {
+    clear_error();
+    if (!det || !img || !phy || !out) return set_error(SM_E_ARG, "null argument");
+    if (n_near < 1 || n_ring < 0 || n_near + n_ring > 256) return set_error(SM_E_ARG, "need 1 <= n_near, n_near + n_ring <= 256");
+    int rc = use_device(device);
+    if (rc < 0) return rc;
+    float col0[110] = {0};
+    auto* S = new SmSurrogate();
+    auto fail = [&](int code) {
+        delete S;
+        return code;
+    };
+    if ((rc = build_geom(col0, det, img, phy, device, rotation, S->G)) < 0) return fail(rc);
+    const int n = S->G.T.n_det;
+    if (use_bitmap) {
+        if (crystal_words_host) {
+            const long long nw = crystal_words(n);
+            if ((rc = S->bits.alloc(nw)) < 0) return fail(rc);
+            if (hipMemcpy(S->bits.ptr, crystal_words_host, sizeof(unsigned) * nw, hipMemcpyHostToDevice) != hipSuccess)
+                return fail(set_error(SM_E_DEVICE, "bitmap upload failed"));
+        } else if ((rc = make_bitmap(S->G, 0, true, nullptr, false, S->bits, nullptr)) < 0) {
+            return fail(rc);
+        }
+    }
+    S->n_near = n_near;
+    S->n_ring = n_ring;
+    const int n_keep = n_near + n_ring;
+    const int n_cand = std::min(256, n - 1);
+    std::vector<int> cand;
+    nearest_candidates(S->G.T, std::max(1, n_cand), cand);
+    DeviceBuf<int> dcand;
+    if ((rc = dcand.upload(cand)) < 0) return fail(rc);
+    if ((rc = S->keep.alloc((size_t)n * n_keep)) < 0) return fail(rc);
+    if ((rc = S->keep_cnt.alloc(n)) < 0) return fail(rc);
+    ScParams p = S->G.base;
+    p.mode = 0;
+    p.det_begin = 0;
+    p.det_count = n;
+    SurrogateRank k{dcand.ptr, std::max(1, n_cand), n_keep};
+    if (n_cand >= 1) {
+        HIP_TRY(launch_surrogate_rank(p, k, S->G.rows.ptr, S->G.cry.ptr, S->bits.ptr, S->keep.ptr, S->keep_cnt.ptr,
+                                      nullptr));
+    } else {
+        HIP_TRY(hipMemset(S->keep_cnt.ptr, 0, sizeof(int) * n));
+    }
+    HIP_TRY(hipDeviceSynchronize());
+    // per-row feature constants
+    float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
+    double cen[3] = {0, 0, 0};
+    for (const Crystal& c : S->G.T.crystals) {
+        const float v[3] = {c.cx, c.cy, c.cz};
+        for (int a = 0; a < 3; ++a) {
+            bmin[a] = std::min(bmin[a], v[a]);
+            bmax[a] = std::max(bmax[a], v[a]);
+            cen[a] += v[a];
+        }
+    }
+    for (double& c : cen) c /= n;
+    std::vector<SgRow> rows(n);
+    for (int r = 0; r < n; ++r) {
+        const Crystal& c = S->G.T.crystals[r];
+        SgRow& R = rows[r];
+        R.cx = c.cx;
+        R.cy = c.cy;
+        R.cz = c.cz;
+        R.use = S->G.hrows[r].use;
+        const float v[3] = {c.cx, c.cy, c.cz};
+        float f[3];
+        for (int a = 0; a < 3; ++a) {
+            const float mid = 0.5f * (bmin[a] + bmax[a]), half = std::max(0.5f * (bmax[a] - bmin[a]), 1.0f);
+            f[a] = (v[a] - mid) / half;
+        }
+        R.f3 = f[0];
+        R.f4 = f[1];
+        R.f5 = f[2];
+    }
+    if ((rc = S->srow.upload(rows)) < 0) return fail(rc);
+    SgParams& q = S->q;
+    const ScParams& b = S->G.base;
+    q.X = b.X; q.Y = b.Y; q.Z = b.Z;
+    q.wx = (float)b.wx; q.wy = (float)b.wy; q.wz = (float)b.wz;
+    q.sx = (float)b.sx; q.sy = (float)b.sy; q.sz = (float)b.sz;
+    q.rc = (float)b.rc; q.rs = (float)b.rs;
+    q.icx = (float)(b.sx * b.rc - b.sy * b.rs);
+    q.icy = (float)(b.sx * b.rs + b.sy * b.rc);
+    q.icz = (float)b.sz;
+    q.irho = (float)(0.5 * std::sqrt((double)b.X * b.X * b.wx * b.wx + (double)b.Y * b.Y * b.wy * b.wy +
+                                     (double)b.Z * b.Z * b.wz * b.wz));
+    const double ddx = cen[0] - q.icx, ddy = cen[1] - q.icy, ddz = cen[2] - q.icz;
+    q.d_ref = (float)std::max(1.0, std::sqrt(ddx * ddx + ddy * ddy + ddz * ddz));
+    q.V = b.V;
+    q.n_det = n;
+    q.n_near = n_near;
+    q.n_keep = n_keep;
+    q.pe_scale = 1.0f;
+    q.s_scale = 1.0f;
+    q.mu_y = 0.0f;
+    q.sig_y = 1.0f;
+    q.phi = 0.0f;
+    q.n_tiles = (b.V + 63) / 64;
+    *out = S;
+    return SM_OK;
+}
+
+int sm_surrogate_set_norm(SmSurrogate* S, float pe_scale, float s_scale, float mu_y, float sig_y, float phi) {
+    clear_error();
+    if (!S || !(pe_scale > 0.0f) || !(s_scale > 0.0f) || !(sig_y > 0.0f) || !(phi >= 0.0f))
+        return set_error(SM_E_ARG, "bad normalisation");
+    S->q.pe_scale = pe_scale;
+    S->q.s_scale = s_scale;
+    S->q.mu_y = mu_y;
+    S->q.sig_y = sig_y;
+    S->q.phi = phi;
+    return SM_OK;
+}
+
+int sm_surrogate_row_sites(const SmSurrogate* S, int row, int* sites, float* weights, int cap) {
+    clear_error();
+    if (!S || row < 0 || row >= S->G.T.n_det) return set_error(SM_E_ARG, "bad row");
+    const int nk = S->q.n_keep;
+    int cnt = 0;
+    HIP_TRY(hipMemcpy(&cnt, S->keep_cnt.ptr + row, sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<ScPair> P(std::max(1, cnt));
+    if (cnt > 0)
+        HIP_TRY(hipMemcpy(P.data(), S->keep.ptr + (long long)row * nk, sizeof(ScPair) * cnt, hipMemcpyDeviceToHost));
+    for (int i = 0; i < cnt && i < cap; ++i) {
+        if (sites) sites[i] = P[i].site;
+        if (weights) weights[i] = P[i].pad;
+    }
+    return cnt;
+}
+
+int sm_surrogate_near(SmSurrogate* S, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
+                      long long row_stride, void* stream) {
+    clear_error();
+    if (!S || !pe_dev || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    int rc = check_rows(S, det_begin, det_count);
+    if (rc < 0) return rc;
+    if ((rc = use_device(S->G.device)) < 0) return rc;
+    ScParams p = S->G.base;
+    p.mode = 0;
+    p.det_begin = det_begin;
+    p.det_count = det_count;
+    p.max_sites = S->q.n_keep;
+    p.site_cap = S->n_near;
+    p.row_stride = row_stride > 0 ? row_stride : p.V;
+    p.n_tiles = (int)((p.V + 63) / 64);
+    p.n_rgroups = (det_count + 3) / 4;
+    p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
+    HIP_TRY(launch_scatter(p, S->G.rows.ptr, S->keep.ptr + (long long)det_begin * S->q.n_keep,
+                           S->keep_cnt.ptr + det_begin, pe_dev, dst_dev, (hipStream_t)stream));
+    return SM_OK;
+}
+
+int sm_surrogate_exact_rows(SmSurrogate* S, const float* pe_dev, const int* rows, int n_rows, float* dst_dev,
+                            long long row_stride, void* stream) {
+    clear_error();
+    if (!S || !pe_dev || !rows || !dst_dev || n_rows <= 0) return set_error(SM_E_ARG, "bad argument");
+    for (int i = 0; i < n_rows; ++i)
+        if (rows[i] < 0 || rows[i] >= S->G.T.n_det) return set_error(SM_E_ARG, "row outside the detector table");
+    int rc = use_device(S->G.device);
+    if (rc < 0) return rc;
+    std::vector<int> list(rows, rows + n_rows);
+    return run_scatter_list(S->G, 0, S->bits.ptr, pe_dev, list, 0, false, nullptr, dst_dev,
+                            row_stride > 0 ? row_stride : S->G.base.V, (hipStream_t)stream);
+}
+
+int sm_surrogate_features(SmSurrogate* S, const float* pe_dev, const float* near_dev, long long near_stride,
+                          int det_begin, int det_count, const int* vox_dev, int n_vox, float* x_dev, void* stream) {
+    clear_error();
+    if (!S || !pe_dev || !near_dev || !x_dev || n_vox <= 0) return set_error(SM_E_ARG, "bad argument");
+    int rc = check_rows(S, det_begin, det_count);
+    if (rc < 0) return rc;
+    if ((rc = use_device(S->G.device)) < 0) return rc;
+    SgParams q = S->q;
+    q.det_begin = det_begin;
+    q.det_count = det_count;
+    HIP_TRY(launch_surrogate_features(q, S->srow.ptr, S->keep.ptr, S->keep_cnt.ptr, pe_dev, near_dev,
+                                      near_stride > 0 ? near_stride : q.V, vox_dev, n_vox, x_dev,
+                                      (hipStream_t)stream));
+    return SM_OK;
+}
+
+int sm_surrogate_apply(SmSurrogate* S, const void* packed_dev, int n_members, const float* pe_dev, int det_begin,
+                       int det_count, float* dst_dev, long long row_stride, float tol, float floor_abs,
+                       unsigned char* tile_flags_dev, void* stream) {
+    clear_error();
+    if (!S || !packed_dev || !pe_dev || !dst_dev || n_members < 1) return set_error(SM_E_ARG, "bad argument");
+    int rc = check_rows(S, det_begin, det_count);
+    if (rc < 0) return rc;
+    if ((rc = use_device(S->G.device)) < 0) return rc;
+    SgParams q = S->q;
+    q.det_begin = det_begin;
+    q.det_count = det_count;
+    q.row_stride = row_stride > 0 ? row_stride : q.V;
+    q.n_members = n_members;
+    q.tol = tol;
+    q.floor_abs = floor_abs;
+    HIP_TRY(launch_surrogate_apply(q, S->srow.ptr, S->keep.ptr, S->keep_cnt.ptr, (const uint8_t*)packed_dev, pe_dev,
+                                   dst_dev, tile_flags_dev, (hipStream_t)stream));
+    return SM_OK;
+}
+
+long long sm_surrogate_fallback(SmSurrogate* S, const float* pe_dev, int det_begin, int det_count,
+                                const unsigned char* tile_flags_dev, float* dst_dev, long long row_stride,
+                                void* stream) {
+    clear_error();
+    if (!S || !pe_dev || !tile_flags_dev || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    int rc = check_rows(S, det_begin, det_count);
+    if (rc < 0) return rc;
+    if ((rc = use_device(S->G.device)) < 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    const long long nt = S->q.n_tiles;
+    std::vector<unsigned char> f((size_t)det_count * nt);
+    HIP_TRY(hipMemcpyAsync(f.data(), tile_flags_dev, f.size(), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    std::vector<int> rows;
+    std::vector<int2> items;
+    for (int rr = 0; rr < det_count; ++rr) {
+        bool any = false;
+        for (long long t = 0; t < nt; ++t) {
+            if (!f[(size_t)rr * nt + t]) continue;
+            if (!any) {
+                rows.push_back(det_begin + rr);
+                any = true;
+            }
+            items.push_back(make_int2((int)rows.size() - 1, (int)t));
+        }
+    }
+    if (items.empty()) return 0;
+    rc = run_scatter_list(S->G, 0, S->bits.ptr, pe_dev, rows, det_begin, true, &items, dst_dev,
+                          row_stride > 0 ? row_stride : S->q.V, s);
+    if (rc < 0) return rc;
+    return (long long)items.size();
+}
+
+int sm_surrogate_destroy(SmSurrogate* S) {
+    if (S) {
+        use_device(S->G.device);
+        delete S;
+    }
+    return SM_OK;
+}
+
+}  // extern "C"

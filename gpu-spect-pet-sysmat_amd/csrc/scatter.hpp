@@ -45,7 +45,7 @@ struct alignas(16) ScPair {
     float c_fwd;              // cos(asin(R/L)) + margin: "through" test needed when |cos theta| >= this
     int site;                 // crystal index (crystal mode) or hole index (collimator mode)
     float hy1, hy2;           // hole faces (collimator mode, as stored)
-    float pad;
+    float pad;                // surrogate tables: single-site weight at the image centre
 };
 
 struct ScParams {
@@ -66,12 +66,29 @@ struct ScParams {
     int n_tiles;              // voxel tiles of 64
     int n_rgroups;            // detector groups of 4
     long long n_blocks;       // tiles * rgroups (before XCD padding)
+    // optional modes (zero / null = off)
+    const int* row_list;      // prep / accumulate rows: r = row_list[rr], else det_begin + rr
+    int dst_by_row;           // with row_list: 1 = dst row r - det_begin, 0 = dst row rr
+    int site_cap;             // accumulate over the first site_cap pairs of each row only
+    const int2* items;        // list mode: wave w of block b handles items[4b + w] = (rr, voxel tile)
+    long long n_items;
+};
+
+// Scatter-surrogate near field: per detector the n_near + n_ring sites with the
+// largest single-site weight at the image centre, best first; pair.pad holds the
+// weight (crystal mode).
+struct SurrogateRank {
+    const int* cand;          // [n_det][n_cand] candidate sites (nearest crystals), -1 = none
+    int n_cand;               // <= 256
+    int n_keep;               // n_near + n_ring
 };
 
 hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Crystal* cry, const Hole* hol,
                                const unsigned* bits, ScPair* pairs, int* counts, hipStream_t stream);
 hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pairs, const int* counts,
                           const float* pe, float* dst, hipStream_t stream);
+hipError_t launch_surrogate_rank(const ScParams& p, const SurrogateRank& k, const ScRow* rows, const Crystal* cry,
+                                 const unsigned* bits, ScPair* keep, int* counts, hipStream_t stream);
 hipError_t launch_bitmap_crystal(const Crystal* cry, int n, unsigned* words, hipStream_t stream);
 hipError_t launch_bitmap_collimator(const Hole* hol, int n_holes, const Crystal* cry, int n, unsigned* words,
                                     hipStream_t stream);

@@ -34,6 +34,58 @@ __device__ inline float half_diag(const Crystal& K) {  // sqrt(w^2 + h^2 + t^2) 
     return __fdiv_rn(K.rbound, 2.0f);
 }
 
+// Attenuator criteria of the GeometryRelationShip bitmaps (angular bounding
+// spheres; coordinates without FOV2C).  Shared by the bitmap builders and the
+// bitmap-free pair preparation, so both decide identically.
+struct CrossIJ {
+    float xi, yi, zi, xj, yj, zj, cij, cji, uijx, uijy, uijz, ujix, ujiy, ujiz;
+};
+__device__ inline CrossIJ cross_ij(const Crystal& I, const Crystal& J) {
+    CrossIJ c;
+    c.xi = I.cx; c.yi = I.y_raw; c.zi = I.cz;
+    c.xj = J.cx; c.yj = J.y_raw; c.zj = J.cz;
+    const float Ri = half_diag(I), Rj = half_diag(J);
+    const float Lij = dist3(c.xi, c.yi, c.zi, c.xj, c.yj, c.zj);
+    c.cij = __fdiv_rn(Ri, Lij);
+    c.cji = __fdiv_rn(Rj, Lij);
+    c.uijx = __fdiv_rn(c.xi - c.xj, Lij); c.uijy = __fdiv_rn(c.yi - c.yj, Lij); c.uijz = __fdiv_rn(c.zi - c.zj, Lij);
+    c.ujix = __fdiv_rn(c.xj - c.xi, Lij); c.ujiy = __fdiv_rn(c.yj - c.yi, Lij); c.ujiz = __fdiv_rn(c.zj - c.zi, Lij);
+    return c;
+}
+// crystal K between crystals i < j (scatter.cu:1460-1530)
+__device__ inline bool cross_k(const CrossIJ& c, const Crystal& K) {
+    const float xk = K.cx, yk = K.y_raw, zk = K.cz;
+    const float Rk = half_diag(K);
+    const float Lik = dist3(xk, yk, zk, c.xi, c.yi, c.zi);
+    const float cki = __fdiv_rn(Rk, Lik);
+    const float Ljk = dist3(xk, yk, zk, c.xj, c.yj, c.zj);
+    const float ckj = __fdiv_rn(Rk, Ljk);
+    const float ukix = __fdiv_rn(xk - c.xi, Lik), ukiy = __fdiv_rn(yk - c.yi, Lik), ukiz = __fdiv_rn(zk - c.zi, Lik);
+    const float ukjx = __fdiv_rn(xk - c.xj, Ljk), ukjy = __fdiv_rn(yk - c.yj, Ljk), ukjz = __fdiv_rn(zk - c.zj, Ljk);
+    if (dist3(ukix, ukiy, ukiz, c.ujix, c.ujiy, c.ujiz) <= __fadd_rn(cki, c.cji)) return true;
+    return dist3(ukjx, ukjy, ukjz, c.uijx, c.uijy, c.uijz) <= __fadd_rn(ckj, c.cij);
+}
+struct CrossHJ {
+    float xh, yh, zh, cji, ujix, ujiy, ujiz;
+};
+__device__ inline CrossHJ cross_hj(const Hole& H, const Crystal& J) {
+    CrossHJ c;
+    c.xh = H.x; c.yh = __fdiv_rn(__fadd_rn(H.y1_raw, H.y2_raw), 2.0f); c.zh = H.z;
+    const float Rj = half_diag(J);
+    const float Lij = dist3(c.xh, c.yh, c.zh, J.cx, J.y_raw, J.cz);
+    c.cji = __fdiv_rn(Rj, Lij);
+    c.ujix = __fdiv_rn(J.cx - c.xh, Lij); c.ujiy = __fdiv_rn(J.y_raw - c.yh, Lij); c.ujiz = __fdiv_rn(J.cz - c.zh, Lij);
+    return c;
+}
+// crystal K between layer-0 hole h and crystal j (scatter.cu:1960-2000)
+__device__ inline bool cross_hk(const CrossHJ& c, const Crystal& K) {
+    const float Rk = half_diag(K);
+    const float Lik = dist3(K.cx, K.y_raw, K.cz, c.xh, c.yh, c.zh);
+    const float cki = __fdiv_rn(Rk, Lik);
+    const float ukix = __fdiv_rn(K.cx - c.xh, Lik), ukiy = __fdiv_rn(K.y_raw - c.yh, Lik), ukiz = __fdiv_rn(K.cz - c.zh, Lik);
+    return dist3(ukix, ukiy, ukiz, c.ujix, c.ujiy, c.ujiz) <= __fadd_rn(cki, c.cji);
+}
+
 // One thread = crystal pair (i < j) x 32 consecutive k.
 __global__ void __launch_bounds__(256) bitmap_crystal_kernel(const Crystal* __restrict__ cry, int n,
                                                              unsigned* __restrict__ words) {
@@ -44,29 +96,12 @@ __global__ void __launch_bounds__(256) bitmap_crystal_kernel(const Crystal* __re
     if (t >= njs) return;
     const int j = i + 1 + (int)(t / nchunk);
     const int k0 = (int)(t % nchunk) * 32;
-    const Crystal I = cry[i], J = cry[j];
-    const float xi = I.cx, yi = I.y_raw, zi = I.cz, xj = J.cx, yj = J.y_raw, zj = J.cz;
-    const float Ri = half_diag(I), Rj = half_diag(J);
-    const float Lij = dist3(xi, yi, zi, xj, yj, zj);
-    const float cij = __fdiv_rn(Ri, Lij), cji = __fdiv_rn(Rj, Lij);
-    const float uijx = __fdiv_rn(xi - xj, Lij), uijy = __fdiv_rn(yi - yj, Lij), uijz = __fdiv_rn(zi - zj, Lij);
-    const float ujix = __fdiv_rn(xj - xi, Lij), ujiy = __fdiv_rn(yj - yi, Lij), ujiz = __fdiv_rn(zj - zi, Lij);
+    const CrossIJ c = cross_ij(cry[i], cry[j]);
     unsigned bits = 0;
     for (int b = 0; b < 32; ++b) {
         const int k = k0 + b;
         if (k >= n || k == i || k == j) continue;
-        const Crystal K = cry[k];
-        const float xk = K.cx, yk = K.y_raw, zk = K.cz;
-        const float Rk = half_diag(K);
-        const float Lik = dist3(xk, yk, zk, xi, yi, zi);
-        const float cki = __fdiv_rn(Rk, Lik);
-        const float Ljk = dist3(xk, yk, zk, xj, yj, zj);
-        const float ckj = __fdiv_rn(Rk, Ljk);
-        const float ukix = __fdiv_rn(xk - xi, Lik), ukiy = __fdiv_rn(yk - yi, Lik), ukiz = __fdiv_rn(zk - zi, Lik);
-        const float ukjx = __fdiv_rn(xk - xj, Ljk), ukjy = __fdiv_rn(yk - yj, Ljk), ukjz = __fdiv_rn(zk - zj, Ljk);
-        bool cross = dist3(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= __fadd_rn(cki, cji);
-        if (!cross) cross = dist3(ukjx, ukjy, ukjz, uijx, uijy, uijz) <= __fadd_rn(ckj, cij);
-        if (cross) bits |= 1u << b;
+        if (cross_k(c, cry[k])) bits |= 1u << b;
     }
     if (!bits) return;
     const long long pair = (long long)i * n - ((long long)i * (i + 1)) / 2 + j;
@@ -90,23 +125,12 @@ __global__ void __launch_bounds__(256) bitmap_collimator_kernel(const Hole* __re
     if (t >= (long long)n * nchunk) return;
     const int j = (int)(t / nchunk);
     const int k0 = (int)(t % nchunk) * 32;
-    const Hole H = hol[h];
-    const float xh = H.x, yh = __fdiv_rn(__fadd_rn(H.y1_raw, H.y2_raw), 2.0f), zh = H.z;
-    const Crystal J = cry[j];
-    const float Rj = half_diag(J);
-    const float Lij = dist3(xh, yh, zh, J.cx, J.y_raw, J.cz);
-    const float cji = __fdiv_rn(Rj, Lij);
-    const float ujix = __fdiv_rn(J.cx - xh, Lij), ujiy = __fdiv_rn(J.y_raw - yh, Lij), ujiz = __fdiv_rn(J.cz - zh, Lij);
+    const CrossHJ c = cross_hj(hol[h], cry[j]);
     unsigned bits = 0;
     for (int b = 0; b < 32; ++b) {
         const int k = k0 + b;
         if (k >= n) continue;
-        const Crystal K = cry[k];
-        const float Rk = half_diag(K);
-        const float Lik = dist3(K.cx, K.y_raw, K.cz, xh, yh, zh);
-        const float cki = __fdiv_rn(Rk, Lik);
-        const float ukix = __fdiv_rn(K.cx - xh, Lik), ukiy = __fdiv_rn(K.y_raw - yh, Lik), ukiz = __fdiv_rn(K.cz - zh, Lik);
-        if (dist3(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= __fadd_rn(cki, cji)) bits |= 1u << b;
+        if (cross_hk(c, cry[k])) bits |= 1u << b;
     }
     if (!bits) return;
     const long long bit0 = (long long)h * n * n + (long long)j * n + k0;
@@ -140,8 +164,109 @@ __device__ inline bool test_bit(const unsigned* __restrict__ w, long long bit) {
     return (w[bit >> 5] >> (bit & 31)) & 1u;
 }
 
+// Image bounding sphere (rotated centre, radius + 1e-3) used to prune pairs.
+struct ImageSphere {
+    double cx, cy, cz, rho;
+};
+__device__ inline ImageSphere image_sphere(const ScParams& p) {
+    ImageSphere b;
+    b.cx = p.sx * p.rc - p.sy * p.rs;
+    b.cy = p.sx * p.rs + p.sy * p.rc;
+    b.cz = p.sz;
+    b.rho = 0.5 * sqrt((double)p.X * p.X * p.wx * p.wx + (double)p.Y * p.Y * p.wy * p.wy +
+                       (double)p.Z * p.Z * p.wz * p.wz) + 1e-3;
+    return b;
+}
+
+// Per-(detector r, site s) constants.  Returns false when the pair cannot
+// contribute for any voxel (dead site, or scatter angle outside the window band
+// over the whole image).  Attenuators come from the bitmap when `bits` is given,
+// else from the same criterion evaluated inline.
+__device__ inline bool make_pair(const ScParams& p, const ScRow& R, int r, int s, const Crystal* __restrict__ cry,
+                                 const Hole* __restrict__ hol, const unsigned* __restrict__ bits,
+                                 const ImageSphere& B, ScPair& P) {
+    const int n = p.n_det;
+    P = ScPair{};
+    double sx, sy, sz;
+    bool alive;
+    if (p.mode == 0) {
+        const Crystal K = cry[s];
+        sx = K.cx; sy = ycen(K, p.fov2c); sz = K.cz;
+        alive = K.mu_t > 0.01f;  // a crystal with mu_t <= 0.01 has an all-zero PE row
+    } else {
+        const Hole H = hol[s];
+        sx = H.x;
+        sy = ((double)H.y1_raw + (double)H.y2_raw) / 2.0 + p.fov2c;
+        sz = H.z;
+        alive = true;
+        P.hy1 = H.y1_raw;
+        P.hy2 = H.y2_raw;
+    }
+    const double bx = R.cx - sx, by = R.cy - sy, bz = R.cz - sz;
+    const double L = sqrt(bx * bx + by * by + bz * bz);
+    const double ax = sx - B.cx, ay = sy - B.cy, az = sz - B.cz;
+    const double la = sqrt(ax * ax + ay * ay + az * az);
+    if (alive && la > B.rho) {
+        const double c0 = (ax * bx + ay * by + az * bz) / (la * L);
+        const double th0 = acos(fmin(1.0, fmax(-1.0, c0)));
+        const double dth = asin(fmin(1.0, B.rho / la));
+        if (th0 - dth > R.th_hi + 1e-6 || th0 + dth < R.th_lo - 1e-6) alive = false;
+    }
+    if (!alive) return false;
+    P.site = s;
+    P.sx = sx; P.sy = sy; P.sz = sz;
+    P.L = L;
+    P.bx = bx / L; P.by = by / L; P.bz = bz / L;
+    const double ssx = (sx - R.cx) * R.crn - (sz - R.cz) * R.srn, ssy = sy - R.cy;
+    const double ssz = (sx - R.cx) * R.srn + (sz - R.cz) * R.crn;
+    P.ssx = ssx; P.ssy = ssy; P.ssz = ssz;
+    for (int c = 0; c < 8; ++c) {
+        const double qx = (c & 1) ? R.hx : -R.hx, qy = (c & 2) ? R.hy : -R.hy, qz = (c & 4) ? R.hz : -R.hz;
+        const double ex = qx - ssx, ey = qy - ssy, ez = qz - ssz;
+        P.ib[c] = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
+    }
+    // absorption chord: S_self -> Q_self + 1000 (Q_self - S_self) / L, Q_self = 0
+    const double tx = -1000.0 * ssx / L, ty = -1000.0 * ssy / L, tz = -1000.0 * ssz / L;
+    P.l_abs = (float)chord_box64(ssx, ssy, ssz, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
+    const double q = R.rdet / L;
+    P.c_fwd = (q >= 1.0) ? -2.0f : (float)(sqrt(1.0 - q * q) - 1e-6);
+    // attenuators (scatter.cu:1343-1401 / 1839-1897)
+    double ape = 0.0, ac = 0.0;
+    const int i0 = s < r ? s : r, j0 = s < r ? r : s;
+    CrossIJ cij{};
+    CrossHJ chj{};
+    if (!bits) {
+        if (p.mode == 0) cij = cross_ij(cry[i0], cry[j0]);
+        else chj = cross_hj(hol[s], cry[r]);
+    }
+    for (int a = 0; a < n; ++a) {
+        if (a == r) continue;
+        bool hit;
+        if (p.mode == 0) {
+            if (a == s) continue;
+            if (bits) {
+                hit = test_bit(bits, ((long long)i0 * n - ((long long)i0 * (i0 + 1)) / 2 + j0) * n + a);
+            } else {
+                hit = cross_k(cij, cry[a]);
+            }
+        } else {
+            hit = bits ? test_bit(bits, (long long)s * n * n + (long long)r * n + a) : cross_hk(chj, cry[a]);
+        }
+        if (!hit) continue;
+        const Crystal A = cry[a];
+        const double c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
+        ape += c * (double)A.mu_pe;
+        ac += c * (double)A.mu_c;
+    }
+    P.A_pe = (float)ape;
+    P.A_c = (float)ac;
+    return true;
+}
+
+__device__ inline int prep_row(const ScParams& p, int rr) { return p.row_list ? p.row_list[rr] : p.det_begin + rr; }
+
 // One workgroup per detecting crystal r; lanes evaluate sites, survivors are
-// compacted in index order into pairs[r * max_sites + slot].
+// compacted in index order into pairs[rr * max_sites + slot].
 __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScRow* __restrict__ rows,
                                                            const Crystal* __restrict__ cry,
                                                            const Hole* __restrict__ hol,
@@ -149,89 +274,18 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
                                                            ScPair* __restrict__ pairs, int* __restrict__ counts) {
     __shared__ int s_wcount[4];
     const int rr = blockIdx.x;
-    const int r = p.det_begin + rr;
+    const int r = prep_row(p, rr);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const ScRow R = rows[r];
-    const int n = p.n_det;
-    const int n_sites = (p.mode == 0) ? n : p.max_sites;
-    // image bounding sphere (rotated centre)
-    const double icx0 = p.sx, icy0 = p.sy;
-    const double icx = icx0 * p.rc - icy0 * p.rs, icy = icx0 * p.rs + icy0 * p.rc, icz = p.sz;
-    const double rho = 0.5 * sqrt((double)p.X * p.X * p.wx * p.wx + (double)p.Y * p.Y * p.wy * p.wy +
-                                  (double)p.Z * p.Z * p.wz * p.wz) + 1e-3;
+    const int n_sites = (p.mode == 0) ? p.n_det : p.max_sites;
+    const ImageSphere B = image_sphere(p);
     int count = 0;
     if (R.use != 0.0f) {
         for (int base = 0; base < n_sites; base += 256) {
             const int s = base + tid;
-            bool alive = false;
-            ScPair P{};
-            if (s < n_sites && !(p.mode == 0 && s == r)) {
-                double sx, sy, sz;
-                if (p.mode == 0) {
-                    const Crystal K = cry[s];
-                    sx = K.cx; sy = ycen(K, p.fov2c); sz = K.cz;
-                    alive = K.mu_t > 0.01f;  // a crystal with mu_t <= 0.01 has an all-zero PE row
-                } else {
-                    const Hole H = hol[s];
-                    sx = H.x;
-                    sy = ((double)H.y1_raw + (double)H.y2_raw) / 2.0 + p.fov2c;
-                    sz = H.z;
-                    alive = true;
-                    P.hy1 = H.y1_raw;
-                    P.hy2 = H.y2_raw;
-                }
-                const double bx = R.cx - sx, by = R.cy - sy, bz = R.cz - sz;
-                const double L = sqrt(bx * bx + by * by + bz * bz);
-                // pairs whose scatter angle is outside the window band for every voxel
-                const double ax = sx - icx, ay = sy - icy, az = sz - icz;
-                const double la = sqrt(ax * ax + ay * ay + az * az);
-                if (alive && la > rho) {
-                    const double c0 = (ax * bx + ay * by + az * bz) / (la * L);
-                    const double th0 = acos(fmin(1.0, fmax(-1.0, c0)));
-                    const double dth = asin(fmin(1.0, rho / la));
-                    if (th0 - dth > R.th_hi + 1e-6 || th0 + dth < R.th_lo - 1e-6) alive = false;
-                }
-                if (alive) {
-                    P.site = s;
-                    P.sx = sx; P.sy = sy; P.sz = sz;
-                    P.L = L;
-                    P.bx = bx / L; P.by = by / L; P.bz = bz / L;
-                    const double ssx = (sx - R.cx) * R.crn - (sz - R.cz) * R.srn, ssy = sy - R.cy;
-                    const double ssz = (sx - R.cx) * R.srn + (sz - R.cz) * R.crn;
-                    P.ssx = ssx; P.ssy = ssy; P.ssz = ssz;
-                    for (int c = 0; c < 8; ++c) {
-                        const double qx = (c & 1) ? R.hx : -R.hx, qy = (c & 2) ? R.hy : -R.hy, qz = (c & 4) ? R.hz : -R.hz;
-                        const double ex = qx - ssx, ey = qy - ssy, ez = qz - ssz;
-                        P.ib[c] = 1.0 / sqrt(ex * ex + ey * ey + ez * ez);
-                    }
-                    // absorption chord: S_self -> Q_self + 1000 (Q_self - S_self) / L, Q_self = 0
-                    const double tx = -1000.0 * ssx / L, ty = -1000.0 * ssy / L, tz = -1000.0 * ssz / L;
-                    P.l_abs = (float)chord_box64(ssx, ssy, ssz, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
-                    const double q = R.rdet / L;
-                    P.c_fwd = (q >= 1.0) ? -2.0f : (float)(sqrt(1.0 - q * q) - 1e-6);
-                    // attenuators listed by the bitmap (scatter.cu:1343-1401 / 1839-1897)
-                    double ape = 0.0, ac = 0.0;
-                    for (int a = 0; a < n; ++a) {
-                        if (a == r) continue;
-                        long long bit;
-                        if (p.mode == 0) {
-                            if (a == s) continue;
-                            const int i0 = s < r ? s : r, j0 = s < r ? r : s;
-                            bit = ((long long)i0 * n - ((long long)i0 * (i0 + 1)) / 2 + j0) * n + a;
-                        } else {
-                            bit = (long long)s * n * n + (long long)r * n + a;
-                        }
-                        if (!test_bit(bits, bit)) continue;
-                        const Crystal A = cry[a];
-                        const double c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
-                        ape += c * (double)A.mu_pe;
-                        ac += c * (double)A.mu_c;
-                    }
-                    P.A_pe = (float)ape;
-                    P.A_c = (float)ac;
-                }
-            }
+            ScPair P;
+            const bool alive = s < n_sites && !(p.mode == 0 && s == r) && make_pair(p, R, r, s, cry, hol, bits, B, P);
             const unsigned long long bal = __ballot(alive);
             if (lane == 0) s_wcount[wave] = __popcll(bal);
             __syncthreads();
@@ -261,117 +315,208 @@ __device__ inline float window_probability(double e, double res, double lo, doub
     return 0.5f * (erff(z2) - erff(z1));
 }
 
+// Voxel point of element v, world and crystal-r frames (scatter.cu:1153-1166),
+// FP64 from the FP32 parameters.
+struct VoxelPt {
+    double xI, yI, zI, xIs, yIs, zIs;
+};
+__device__ inline VoxelPt voxel_point(const ScParams& p, const ScRow& R, double x0, double y0, double zI) {
+    VoxelPt q;
+    q.xI = x0 * p.rc - y0 * p.rs;
+    q.yI = x0 * p.rs + y0 * p.rc;
+    q.zI = zI;
+    q.xIs = (q.xI - R.cx) * R.crn - (q.zI - R.cz) * R.srn;
+    q.yIs = q.yI - R.cy;
+    q.zIs = (q.xI - R.cx) * R.srn + (q.zI - R.cz) * R.crn;
+    return q;
+}
+__device__ inline VoxelPt voxel_of(const ScParams& p, const ScRow& R, long long v) {
+    const int X = p.X, Y = p.Y;
+    const int iz = (int)(v / ((long long)X * Y));
+    const int rem = (int)(v % ((long long)X * Y));
+    const int iy = rem / X, ix = rem % X;
+    const double x0 = ((double)ix - (double)X / 2.0 + 0.5) * p.wx + p.sx;
+    const double y0 = ((double)iy - (double)Y / 2.0 + 0.5) * p.wy + p.sy;
+    const double zI = ((double)iz - (double)p.Z / 2.0 + 0.5) * p.wz + p.sz;
+    return voxel_point(p, R, x0, y0, zI);
+}
+
+// Source weight of a site for the voxel: PE[site, v] mu_c/mu_pe (crystal mode,
+// pe_v = PE[site, v]) or the virtual-disc solid angle of the hole.
+__device__ inline double site_weight(const ScParams& p, const ScRow& R, const ScPair& S, const VoxelPt& q,
+                                     float pe_v) {
+    if (p.mode == 0) return (double)(pe_v * (float)(R.mu_c / R.mu_pe));
+    const double lx = q.xI - S.sx, ly = q.yI - S.sy, lz = q.zI - S.sz;
+    const double Li = sqrt(lx * lx + ly * ly + lz * lz);
+    const double dyh = (double)S.hy2 - (double)S.hy1;
+    double solid;
+    if (p.mode == 1) {
+        const double vr = (double)0.1f;
+        solid = (vr * vr * M_PI + dyh * vr) / (4.0 * M_PI * Li * Li) * fabs(S.sy - q.yI) / Li;
+    } else {
+        const double vr = (double)0.2f;
+        solid = (vr * vr + dyh * vr) / (4.0 * Li * Li);
+    }
+    return solid * p.w_col;
+}
+
+// Contribution of one site to element (r, voxel) with source weight w
+// (scatter.cu:1240-1427 / 1736-1923).
+__device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPair& S, const VoxelPt& q,
+                                   const KnCoef& kc, double w) {
+    const double e0 = p.e0, ea = p.e0 / 511.0;
+    const double two_pi = 2.0 * M_PI;
+    // cone angle at the site (scatter.cu:996-1015, 1246-1248)
+    const double ax = S.sx - q.xI, ay = S.sy - q.yI, az = S.sz - q.zI;
+    const double ma = sqrt(ax * ax + ay * ay + az * az);
+    const double ima = 1.0 / ma;
+    double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
+    c = fmin(1.0, fmax(-1.0, c));
+    const double es = e0 / (1.0 + ea * (1.0 - c));
+    const double res_s = R.res * sqrt(es / e0);
+    const double k2 = 2.0 * res_s / (double)2.35482f;
+    if ((es * (1.0 + k2)) <= R.lo || (es * (1.0 - k2)) >= R.hi) return 0.0;
+    const float p_e = window_probability(es, res_s, R.lo, R.hi);
+    const double sn = sqrt(fmax(0.0, 1.0 - c * c));
+    float phi;
+    if (S.L * sn * 2.0 <= R.rdet)
+        phi = (float)two_pi;
+    else
+        phi = 4.0f * asinf(fminf((float)(R.rdet / 2.0 / S.L / sn), 1.0f));
+    // theta range over the corners of r (scatter.cu:1303-1336)
+    const double asx = S.ssx - q.xIs, asy = S.ssy - q.yIs, asz = S.ssz - q.zIs;
+    const double d0 = asx * S.ssx + asy * S.ssy + asz * S.ssz;
+    const double qx = R.hx * asx, qy = R.hy * asy, qz = R.hz * asz;
+    double cmin = 2.0, cmax = -2.0;
+#pragma unroll
+    for (int k = 0; k < 8; ++k) {
+        const double dq = ((k & 1) ? qx : -qx) + ((k & 2) ? qy : -qy) + ((k & 4) ? qz : -qz) - d0;
+        const double cq = fmin(1.0, fmax(-1.0, dq * S.ib[k] * ima));
+        cmin = fmin(cmin, cq);
+        cmax = fmax(cmax, cq);
+    }
+    double th_min = acos(cmax);
+    const double th_max = acos(cmin);
+    if (c >= (double)S.c_fwd || c <= -(double)S.c_fwd) {
+        // the extended image -> site line through crystal r (scatter.cu:1298-1301, 1332-1335)
+        const double f = 1000.0 * ima;
+        const double tx = S.ssx + f * asx, ty = S.ssy + f * asy, tz = S.ssz + f * asz;
+        const double through = chord_box64(q.xIs, q.yIs, q.zIs, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
+        if (through > (double)0.001f) th_min = 0.0;
+    }
+    const int nstep = (int)((th_max - th_min) / p.h);
+    const double ratio = kn_sum(kc, th_min, nstep, p.h) / p.kn_total;
+    // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
+    const double x = e0 / es, x3 = x * x * x;
+    const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
+    const double pe_abs = R.mu_pe * x3, tot_abs = pe_abs + R.mu_c * x;
+    const float absorb = -expm1f(-(float)((double)S.l_abs * tot_abs));
+    return w * ((double)phi / two_pi) * ratio * (double)p_e * (double)expf(-att) * (double)absorb *
+           (pe_abs / tot_abs);
+}
+
 __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* __restrict__ rows,
                                                       const ScPair* __restrict__ pairs,
                                                       const int* __restrict__ counts, const float* __restrict__ pe,
                                                       float* __restrict__ dst) {
-    // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
-    // each XCD a contiguous range of (voxel tile, detector group) work where
-    // consecutive groups share the voxel tile (its PE columns stay in that XCD's L2).
-    const long long nb = p.n_blocks;
-    const long long padded = (nb + 7) / 8 * 8;
-    const long long b = blockIdx.x;
-    const long long idx = (b % 8) * (padded / 8) + b / 8;
-    if (idx >= nb) return;
-    const int tile = (int)(idx / p.n_rgroups);
-    const int rg = (int)(idx % p.n_rgroups);
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int rr = rg * 4 + wave;
-    if (rr >= p.det_count) return;
-    const int r = p.det_begin + rr;
+    int rr, tile;
+    if (p.items) {
+        const long long it = (long long)blockIdx.x * 4 + wave;
+        if (it >= p.n_items) return;
+        const int2 item = p.items[it];
+        rr = item.x;
+        tile = item.y;
+    } else {
+        // XCD-aware order: workgroups are dealt round-robin to the 8 XCDs, so give
+        // each XCD a contiguous range of (voxel tile, detector group) work where
+        // consecutive groups share the voxel tile (its PE columns stay in that XCD's L2).
+        const long long nb = p.n_blocks;
+        const long long padded = (nb + 7) / 8 * 8;
+        const long long b = blockIdx.x;
+        const long long idx = (b % 8) * (padded / 8) + b / 8;
+        if (idx >= nb) return;
+        tile = (int)(idx / p.n_rgroups);
+        rr = (int)(idx % p.n_rgroups) * 4 + wave;
+        if (rr >= p.det_count) return;
+    }
+    const int r = prep_row(p, rr);
+    const long long drow = (p.row_list && p.dst_by_row) ? (long long)(r - p.det_begin) : (long long)rr;
     const long long v = (long long)tile * 64 + lane;
     const bool active = v < p.V;
     const ScRow R = rows[r];
     double acc = 0.0;
     if (R.use != 0.0f && active) {
-        // voxel point (scatter.cu:1153-1166), FP64 from the FP32 parameters
-        const int X = p.X, Y = p.Y;
-        const int iz = (int)(v / ((long long)X * Y));
-        const int rem = (int)(v % ((long long)X * Y));
-        const int iy = rem / X, ix = rem % X;
-        const double x0 = ((double)ix - (double)X / 2.0 + 0.5) * p.wx + p.sx;
-        const double y0 = ((double)iy - (double)Y / 2.0 + 0.5) * p.wy + p.sy;
-        const double zI = ((double)iz - (double)p.Z / 2.0 + 0.5) * p.wz + p.sz;
-        const double xI = x0 * p.rc - y0 * p.rs, yI = x0 * p.rs + y0 * p.rc;
-        const double xIs = (xI - R.cx) * R.crn - (zI - R.cz) * R.srn, yIs = yI - R.cy;
-        const double zIs = (xI - R.cx) * R.srn + (zI - R.cz) * R.crn;
+        const VoxelPt q = voxel_of(p, R, v);
         const KnCoef kc = kn_coef(p.e0);
-        const double e0 = p.e0, ea = p.e0 / 511.0;
-        const double two_pi = 2.0 * M_PI;
-        const float mu_ratio = (float)(R.mu_c / R.mu_pe);
-        const int n_sites = counts[rr];
+        int n_sites = counts[rr];
+        if (p.site_cap > 0 && n_sites > p.site_cap) n_sites = p.site_cap;
         const ScPair* P = pairs + (long long)rr * p.max_sites;
         for (int j = 0; j < n_sites; ++j) {
             const ScPair S = P[j];
-            // source weight
-            double w;
-            if (p.mode == 0) {
-                w = (double)(pe[(long long)S.site * p.V + v] * mu_ratio);
-            } else {
-                const double lx = xI - S.sx, ly = yI - S.sy, lz = zI - S.sz;
-                const double Li = sqrt(lx * lx + ly * ly + lz * lz);
-                const double dyh = (double)S.hy2 - (double)S.hy1;
-                double solid;
-                if (p.mode == 1) {
-                    const double vr = (double)0.1f;
-                    solid = (vr * vr * M_PI + dyh * vr) / (4.0 * M_PI * Li * Li) * fabs(S.sy - yI) / Li;
-                } else {
-                    const double vr = (double)0.2f;
-                    solid = (vr * vr + dyh * vr) / (4.0 * Li * Li);
-                }
-                w = solid * p.w_col;
-            }
-            // cone angle at the site (scatter.cu:996-1015, 1246-1248)
-            const double ax = S.sx - xI, ay = S.sy - yI, az = S.sz - zI;
-            const double ma = sqrt(ax * ax + ay * ay + az * az);
-            const double ima = 1.0 / ma;
-            double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
-            c = fmin(1.0, fmax(-1.0, c));
-            const double es = e0 / (1.0 + ea * (1.0 - c));
-            const double res_s = R.res * sqrt(es / e0);
-            const double k2 = 2.0 * res_s / (double)2.35482f;
-            if ((es * (1.0 + k2)) <= R.lo || (es * (1.0 - k2)) >= R.hi) continue;
-            const float p_e = window_probability(es, res_s, R.lo, R.hi);
-            const double sn = sqrt(fmax(0.0, 1.0 - c * c));
-            float phi;
-            if (S.L * sn * 2.0 <= R.rdet)
-                phi = (float)two_pi;
-            else
-                phi = 4.0f * asinf(fminf((float)(R.rdet / 2.0 / S.L / sn), 1.0f));
-            // theta range over the corners of r (scatter.cu:1303-1336)
-            const double asx = S.ssx - xIs, asy = S.ssy - yIs, asz = S.ssz - zIs;
-            const double d0 = asx * S.ssx + asy * S.ssy + asz * S.ssz;
-            const double qx = R.hx * asx, qy = R.hy * asy, qz = R.hz * asz;
-            double cmin = 2.0, cmax = -2.0;
-#pragma unroll
-            for (int q = 0; q < 8; ++q) {
-                const double dq = ((q & 1) ? qx : -qx) + ((q & 2) ? qy : -qy) + ((q & 4) ? qz : -qz) - d0;
-                const double cq = fmin(1.0, fmax(-1.0, dq * S.ib[q] * ima));
-                cmin = fmin(cmin, cq);
-                cmax = fmax(cmax, cq);
-            }
-            double th_min = acos(cmax);
-            const double th_max = acos(cmin);
-            if (c >= (double)S.c_fwd || c <= -(double)S.c_fwd) {
-                // the extended image -> site line through crystal r (scatter.cu:1298-1301, 1332-1335)
-                const double f = 1000.0 * ima;
-                const double tx = S.ssx + f * asx, ty = S.ssy + f * asy, tz = S.ssz + f * asz;
-                const double through = chord_box64(xIs, yIs, zIs, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
-                if (through > (double)0.001f) th_min = 0.0;
-            }
-            const int nstep = (int)((th_max - th_min) / p.h);
-            const double ratio = kn_sum(kc, th_min, nstep, p.h) / p.kn_total;
-            // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
-            const double x = e0 / es, x3 = x * x * x;
-            const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
-            const double pe_abs = R.mu_pe * x3, tot_abs = pe_abs + R.mu_c * x;
-            const float absorb = -expm1f(-(float)((double)S.l_abs * tot_abs));
-            const double term = w * ((double)phi / two_pi) * ratio * (double)p_e * (double)expf(-att) *
-                                (double)absorb * (pe_abs / tot_abs);
-            acc += term;
+            const double w = site_weight(p, R, S, q, p.mode == 0 ? pe[(long long)S.site * p.V + v] : 0.0f);
+            acc += site_term(p, R, S, q, kc, w);
         }
     }
-    if (active) dst[(long long)rr * p.row_stride + v] = (float)acc;
+    if (active) dst[drow * p.row_stride + v] = (float)acc;
+}
+
+// Scatter-surrogate ranking: one workgroup per detector row; every candidate
+// site's pair constants and its single-site weight at the image centre (unit
+// PE); a bitonic sort in LDS keeps the best n_keep, best first.
+__global__ void __launch_bounds__(256) surrogate_rank_kernel(ScParams p, SurrogateRank k,
+                                                             const ScRow* __restrict__ rows,
+                                                             const Crystal* __restrict__ cry,
+                                                             const unsigned* __restrict__ bits,
+                                                             ScPair* __restrict__ keep, int* __restrict__ counts) {
+    __shared__ float s_key[256];
+    __shared__ int s_idx[256];
+    __shared__ int s_rank[256];
+    const int rr = blockIdx.x;
+    const int r = p.det_begin + rr;
+    const int t = threadIdx.x;
+    const ScRow R = rows[r];
+    const ImageSphere B = image_sphere(p);
+    ScPair P;
+    float key = -1.0f;
+    if (R.use != 0.0f && t < k.n_cand) {
+        const int s = k.cand[(long long)r * k.n_cand + t];
+        if (s >= 0 && s != r && make_pair(p, R, r, s, cry, nullptr, bits, B, P)) {
+            const VoxelPt q = voxel_point(p, R, p.sx, p.sy, p.sz);
+            const double g = site_term(p, R, P, q, kn_coef(p.e0), (double)(float)(R.mu_c / R.mu_pe));
+            key = g > 0.0 ? (float)g : 0.0f;
+            P.pad = key;
+        }
+    }
+    s_key[t] = key;
+    s_idx[t] = t;
+    __syncthreads();
+    // bitonic sort, descending by key (ties: lower candidate index first)
+    for (int size = 2; size <= 256; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const int o = t ^ stride;
+            if (o > t) {
+                const bool desc = (t & size) == 0;
+                const float ka = s_key[t], kb = s_key[o];
+                const int ia = s_idx[t], ib = s_idx[o];
+                const bool a_first = ka > kb || (ka == kb && ia < ib);
+                if (desc != a_first) {
+                    s_key[t] = kb; s_key[o] = ka;
+                    s_idx[t] = ib; s_idx[o] = ia;
+                }
+            }
+            __syncthreads();
+        }
+    }
+    s_rank[s_idx[t]] = t;
+    __syncthreads();
+    const int rank = s_rank[t];
+    if (key >= 0.0f && rank < k.n_keep) keep[(long long)rr * k.n_keep + rank] = P;
+    if (t == 0) {
+        int c = 0;
+        while (c < k.n_keep && c < 256 && s_key[c] >= 0.0f) ++c;
+        counts[rr] = c;
+    }
 }
 
 hipError_t launch_bitmap_crystal(const Crystal* cry, int n, unsigned* words, hipStream_t stream) {
@@ -403,9 +548,23 @@ hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Cryst
 
 hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pairs, const int* counts,
                           const float* pe, float* dst, hipStream_t stream) {
-    if (p.n_blocks <= 0) return hipSuccess;
-    const long long padded = (p.n_blocks + 7) / 8 * 8;
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)padded), dim3(256), 0, stream, p, rows, pairs, counts, pe, dst);
+    long long grid;
+    if (p.items) {
+        if (p.n_items <= 0) return hipSuccess;
+        grid = (p.n_items + 3) / 4;
+    } else {
+        if (p.n_blocks <= 0) return hipSuccess;
+        grid = (p.n_blocks + 7) / 8 * 8;
+    }
+    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)grid), dim3(256), 0, stream, p, rows, pairs, counts, pe, dst);
+    return hipGetLastError();
+}
+
+hipError_t launch_surrogate_rank(const ScParams& p, const SurrogateRank& k, const ScRow* rows, const Crystal* cry,
+                                 const unsigned* bits, ScPair* keep, int* counts, hipStream_t stream) {
+    if (p.det_count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(surrogate_rank_kernel, dim3((unsigned)p.det_count), dim3(256), 0, stream, p, k, rows, cry, bits,
+                       keep, counts);
     return hipGetLastError();
 }
 

@@ -70,6 +70,23 @@ SIGNATURES = {
     "sm_scatter_plan_crystal": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
     "sm_scatter_plan_collimator": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
     "sm_scatter_plan_destroy": (ctypes.c_int, [_VP]),
+    "sm_scatter_plan_create_ex": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
+                                                 ctypes.c_int, ctypes.POINTER(_VP)]),
+    "sm_scatter_plan_crystal_rows": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_surrogate_create": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_int, ctypes.POINTER(_VP)]),
+    "sm_surrogate_set_norm": (ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,
+                                             ctypes.c_float]),
+    "sm_surrogate_row_sites": (ctypes.c_int, [_VP, ctypes.c_int, _VP, _VP, ctypes.c_int]),
+    "sm_surrogate_near": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_surrogate_exact_rows": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_surrogate_features": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_longlong, ctypes.c_int, ctypes.c_int, _VP,
+                                             ctypes.c_int, _VP, _VP]),
+    "sm_surrogate_apply": (ctypes.c_int, [_VP, _VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_int, _VP,
+                                          ctypes.c_longlong, ctypes.c_float, ctypes.c_float, _VP, _VP]),
+    "sm_surrogate_fallback": (ctypes.c_longlong, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_longlong,
+                                                  _VP]),
+    "sm_surrogate_destroy": (ctypes.c_int, [_VP]),
     "sm_csr_count_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,
                                            _VP, _VP, ctypes.POINTER(ctypes.c_longlong), _VP]),
     "sm_csr_fill_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,

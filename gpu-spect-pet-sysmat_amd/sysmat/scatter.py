@@ -91,20 +91,33 @@ def scatter(parameter_Collimator, parameter_Detector, parameter_Image, parameter
 class ScatterPlan:
     """Geometry, bitmaps and tables resident in HBM; row blocks into device tensors."""
 
+    NO_CRYSTAL_BITMAP = 1
+    NO_COLLIMATOR_BITMAP = 2
+
     def __init__(self, geo: Geometry, rotation: int = 0, variant: int = 0, with_collimator: bool = True,
                  crystal_bits: Optional[np.ndarray] = None, collimator_bits: Optional[np.ndarray] = None,
-                 device: int = 0):
+                 device: int = 0, bitmaps: bool = True):
+        """``bitmaps=False`` evaluates the GeometryRelationShip criterion inline instead of
+        building the bitmaps (same results; needed when N^3/2 bits do not fit)."""
         self._lib = load()
         self.geo = geo
         self._keep = [_f32(geo.col), _f32(geo.det), _img(geo.img), _f32(geo.phy)]
         cb = None if crystal_bits is None else np.ascontiguousarray(crystal_bits, np.uint32)
         hb = None if collimator_bits is None else np.ascontiguousarray(collimator_bits, np.uint32)
+        flags = 0 if bitmaps else (self.NO_CRYSTAL_BITMAP | self.NO_COLLIMATOR_BITMAP)
         h = ctypes.c_void_p()
-        check(self._lib.sm_scatter_plan_create(ptr(self._keep[0]) if with_collimator else None, ptr(self._keep[1]),
-                                               ptr(self._keep[2]), ptr(self._keep[3]), rotation, variant,
-                                               None if cb is None else ptr(cb), None if hb is None else ptr(hb),
-                                               device, ctypes.byref(h)))
+        check(self._lib.sm_scatter_plan_create_ex(ptr(self._keep[0]) if with_collimator else None,
+                                                  ptr(self._keep[1]), ptr(self._keep[2]), ptr(self._keep[3]),
+                                                  rotation, variant, flags, None if cb is None else ptr(cb),
+                                                  None if hb is None else ptr(hb), device, ctypes.byref(h)))
         self._h = h
+
+    def crystal_rows(self, pe_dev, rows, dst_dev, row_stride: int = 0, stream=None) -> None:
+        """Crystal scatter of the listed detector rows into dst rows 0..len(rows)-1."""
+        r = np.ascontiguousarray(rows, np.int32)
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        check(self._lib.sm_scatter_plan_crystal_rows(self._h, ptr(pe_dev), ptr(r), len(r), ptr(dst_dev), row_stride,
+                                                     ctypes.c_void_p(s) if s else None))
 
     def crystal(self, pe_dev, det_begin: int, det_count: int, dst_dev, row_stride: int = 0, stream=None) -> None:
         s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))

@@ -1,0 +1,287 @@
+"""Crystal-scatter surrogate: exact near field + learned far field.
+
+The crystal (inter-crystal) Compton scatter element S[r, v] is a sum over
+scatter sites k of PE[k, v] times a geometric / Klein-Nishina / attenuation
+factor (crystalScatterSysMatCuda, ScatterGen_RayTracing_CircularHole/scatter.cu:1077-1437).
+For a detector r a few sites dominate (the crystals in front of it); the
+remainder is smooth in (r, v).  This module
+
+1. ranks, per detector, the sites by their single-site weight at the image
+   centre and sums the best ``n_near`` exactly for every voxel (GPU kernel:
+   the exact per-site term, capped site list);
+2. trains an ensemble of MLPs (the project's BF16 MFMA kernels) on the far
+   field S - S_near of a seeded sample of detector rows computed by the exact
+   kernel;
+3. predicts the far field for every element (fused feature + MLP kernel),
+   recomputes 64-voxel tiles where members disagree with the exact kernel,
+   and judges the result on a disjoint seeded set of held-out rows.
+
+Only device tensors (PyTorch-ROCm) and C ABI calls are used; there is no CPU
+compute path.
+"""
+from __future__ import annotations
+
+import ctypes
+import dataclasses
+import time
+from typing import Dict, List, Optional, Sequence
+
+import numpy as np
+
+from . import mlp
+from ._lib import check, load, ptr
+from .params import Geometry
+
+N_FEATURES = 12
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def _img(img) -> np.ndarray:
+    out = np.zeros(max(100, len(img)), np.float32)
+    out[:len(img)] = img
+    return out
+
+
+def _sh(stream=None):
+    return mlp._stream_handle(stream)
+
+
+def usable_rows(geo: Geometry) -> np.ndarray:
+    """Rows with a crystal-scatter contribution (Compton on, mu_t > 0.01, mu_pe != 0; scatter.cu:1146)."""
+    c = geo.crystals
+    on = int(np.floor(np.float32(geo.phy[0]) + np.float32(0.5))) == 1
+    ok = (c[:, 6] > np.float32(0.01)) & (c[:, 7] != 0)
+    return np.nonzero(ok & on)[0]
+
+
+@dataclasses.dataclass
+class TrainConfig:
+    train_rows: int = 64           # exact rows used for training
+    judge_rows: int = 16           # disjoint held-out rows (the judge)
+    vox_per_row: int = 8192        # sampled voxels per training row (all if V is smaller)
+    members: int = 2
+    steps: int = 1500
+    batch: int = 4096
+    lr: float = 2e-3
+    seed: int = 0
+    phi_rel: float = 1e-4          # log floor relative to the largest training element
+    tol: float = 0.02              # member disagreement that sends a tile to the exact kernel
+    floor_rel: float = 1e-3        # disagreement is only checked above this fraction of the scale
+
+
+class ScatterSurrogate:
+    """Holds the ranked site tables (HBM), the trained members and the normalisation."""
+
+    def __init__(self, geo: Geometry, rotation: int = 0, n_near: int = 8, n_ring: int = 24,
+                 use_bitmap: Optional[bool] = None, device: int = 0):
+        self.L = load()
+        self.geo = geo
+        self.rotation = rotation
+        self.n_near, self.n_ring = n_near, n_ring
+        N = geo.n_det
+        if use_bitmap is None:
+            use_bitmap = N <= 4096  # N^3/2 bits: 537 MB at N = 2048, 275 GB at N = 16384
+        self._keep = [_f32(geo.det), _img(geo.img), _f32(geo.phy)]
+        h = ctypes.c_void_p()
+        check(self.L.sm_surrogate_create(ptr(self._keep[0]), ptr(self._keep[1]), ptr(self._keep[2]), rotation, None,
+                                         1 if use_bitmap else 0, n_near, n_ring, device, ctypes.byref(h)))
+        self._h = h
+        self.members: List[mlp.Mlp] = []
+        self.packed = None
+        self.norm = None
+        self.train_rows = np.zeros(0, np.int64)
+        self.judge_rows = np.zeros(0, np.int64)
+        self.timings: Dict[str, float] = {}
+        self.cfg = TrainConfig()
+
+    # ---- primitives --------------------------------------------------------
+    def row_sites(self, row: int):
+        cap = self.n_near + self.n_ring
+        s = np.zeros(cap, np.int32)
+        w = np.zeros(cap, np.float32)
+        n = check(self.L.sm_surrogate_row_sites(self._h, int(row), ptr(s), ptr(w), cap))
+        return s[:n], w[:n]
+
+    def near(self, pe, det_begin: int, det_count: int, dst, row_stride: int = 0, stream=None):
+        check(self.L.sm_surrogate_near(self._h, ptr(pe), det_begin, det_count, ptr(dst), row_stride, _sh(stream)))
+
+    def exact_rows(self, pe, rows, dst, row_stride: int = 0, stream=None):
+        r = np.ascontiguousarray(rows, np.int32)
+        check(self.L.sm_surrogate_exact_rows(self._h, ptr(pe), ptr(r), len(r), ptr(dst), row_stride, _sh(stream)))
+
+    def features(self, pe, near_rows, det_begin: int, det_count: int, vox=None, n_vox: Optional[int] = None,
+                 stream=None):
+        import torch
+        V = self.geo.n_vox
+        nv = V if vox is None else int(vox.numel())
+        x = torch.empty((det_count * nv, N_FEATURES), dtype=torch.float32, device=pe.device)
+        check(self.L.sm_surrogate_features(self._h, ptr(pe), ptr(near_rows), near_rows.stride(0), det_begin,
+                                           det_count, None if vox is None else ptr(vox), nv, ptr(x), _sh(stream)))
+        return x
+
+    def set_norm(self, pe_scale, s_scale, mu_y, sig_y, phi):
+        self.norm = dict(pe_scale=float(pe_scale), s_scale=float(s_scale), mu_y=float(mu_y), sig_y=float(sig_y),
+                         phi=float(phi))
+        check(self.L.sm_surrogate_set_norm(self._h, *[ctypes.c_float(v) for v in self.norm.values()]))
+
+    # ---- training ----------------------------------------------------------
+    def sample_rows(self, cfg: TrainConfig):
+        rows = usable_rows(self.geo)
+        rng = np.random.default_rng(cfg.seed)
+        perm = rng.permutation(rows)
+        nt = min(cfg.train_rows, len(perm))
+        nj = min(cfg.judge_rows, max(0, len(perm) - nt))
+        return np.sort(perm[:nt]), np.sort(perm[nt:nt + nj])
+
+    def _near_rows(self, pe, rows):
+        import torch
+        V = self.geo.n_vox
+        out = torch.empty((len(rows), V), dtype=torch.float32, device=pe.device)
+        for i, r in enumerate(rows):
+            self.near(pe, int(r), 1, out[i:i + 1])
+        return out
+
+    def train(self, pe, cfg: Optional[TrainConfig] = None, log=None):
+        """Exact rows -> (features, far-field targets) -> ensemble of MLPs, all on the GPU."""
+        import torch
+        cfg = cfg or TrainConfig()
+        self.cfg = cfg
+        dev = pe.device
+        V = self.geo.n_vox
+        t0 = time.perf_counter()
+        self.train_rows, self.judge_rows = self.sample_rows(cfg)
+        rows = self.train_rows
+        if len(rows) == 0:
+            raise ValueError("no detector rows with a scatter contribution")
+        exact = torch.empty((len(rows), V), dtype=torch.float32, device=dev)
+        self.exact_rows(pe, rows, exact)
+        near = self._near_rows(pe, rows)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        s_scale = float(exact.max().item())
+        if not s_scale > 0:
+            raise ValueError("the exact crystal scatter of the training rows is zero")
+        pe_scale = float(pe.max().item())
+        phi = cfg.phi_rel * s_scale
+        # voxel sample per row (seeded)
+        g = torch.Generator(device="cpu").manual_seed(cfg.seed + 1)
+        if cfg.vox_per_row >= V:
+            vox = None
+            nv = V
+        else:
+            vox = torch.randperm(V, generator=g)[:cfg.vox_per_row].sort().values.to(torch.int32).to(dev)
+            nv = cfg.vox_per_row
+        far = (exact - near).clamp_min_(0.0)
+        # provisional normalisation so that features can be computed; target stats follow
+        self.set_norm(pe_scale, s_scale, 0.0, 1.0, phi)
+        xs, ys, ws = [], [], []
+        for i, r in enumerate(rows):
+            xs.append(self.features(pe, near[i:i + 1], int(r), 1, vox))
+            fr = far[i] if vox is None else far[i].index_select(0, vox.long())
+            ex = exact[i] if vox is None else exact[i].index_select(0, vox.long())
+            ys.append(torch.log(fr + phi))
+            ws.append(torch.sqrt((ex + phi) / s_scale))
+        x = torch.cat(xs)
+        y = torch.cat(ys)
+        w = torch.cat(ws)
+        mu_y, sig_y = float(y.mean().item()), float(y.std().item())
+        sig_y = max(sig_y, 1e-6)
+        self.set_norm(pe_scale, s_scale, mu_y, sig_y, phi)
+        t = ((y - mu_y) / sig_y).contiguous()
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        n = x.shape[0]
+        B = min(cfg.batch, (n // 64) * 64) if n >= 64 else 64
+        self.members = []
+        losses = []
+        for m in range(cfg.members):
+            net = mlp.Mlp(seed=cfg.seed * 1000 + 17 + m, device=str(dev))
+            gen = torch.Generator(device=dev).manual_seed(cfg.seed * 7919 + m)
+            for it in range(cfg.steps):
+                idx = torch.randint(0, n, (B,), device=dev, generator=gen)
+                wb = w.index_select(0, idx)
+                wb = (wb / wb.sum()).contiguous()
+                lr = cfg.lr * 0.5 * (1.0 + np.cos(np.pi * it / cfg.steps))  # cosine schedule
+                want = (it == cfg.steps - 1) or (log is not None and it % 500 == 0)
+                loss = net.train_step(x.index_select(0, idx).contiguous(), t.index_select(0, idx).contiguous(), wb,
+                                      lr=max(lr, cfg.lr * 0.01), batch=B, want_loss=want)
+                if want:
+                    lv = float(loss.item())
+                    if it == cfg.steps - 1:
+                        losses.append(lv)
+                    if log:
+                        log(f"member {m} step {it}: weighted loss {lv:.4e}")
+            self.members.append(net)
+        self.packed = mlp.pack_members(self.members)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        self.timings = {"exact_rows_s": t1 - t0, "features_s": t2 - t1, "train_s": t3 - t2, "total_s": t3 - t0,
+                        "pairs": int(n), "final_losses": losses}
+        return self.timings
+
+    # ---- inference ---------------------------------------------------------
+    def apply(self, pe, det_begin: int, det_count: int, dst, row_stride: int = 0, fallback: bool = True,
+              n_members: Optional[int] = None, stream=None) -> int:
+        """Surrogate crystal scatter of rows [det_begin, +det_count) into dst; returns fallback tiles."""
+        import torch
+        if self.packed is None:
+            raise RuntimeError("surrogate not trained")
+        nm = len(self.members) if n_members is None else n_members
+        self.near(pe, det_begin, det_count, dst, row_stride, stream)
+        n_tiles = (self.geo.n_vox + 63) // 64
+        flags = torch.zeros(det_count * n_tiles, dtype=torch.uint8, device=pe.device) if fallback else None
+        floor_abs = self.cfg.floor_rel * self.norm["s_scale"]
+        check(self.L.sm_surrogate_apply(self._h, ptr(self.packed), nm, ptr(pe), det_begin, det_count, ptr(dst),
+                                        row_stride, ctypes.c_float(self.cfg.tol), ctypes.c_float(floor_abs),
+                                        None if flags is None else ptr(flags), _sh(stream)))
+        if not fallback:
+            return 0
+        return int(check(self.L.sm_surrogate_fallback(self._h, ptr(pe), det_begin, det_count, ptr(flags), ptr(dst),
+                                                      row_stride, _sh(stream))))
+
+    def judge(self, pe, rows: Optional[Sequence[int]] = None, fallback: bool = True) -> dict:
+        """Exact vs surrogate on held-out rows: total-sum error and per-element errors."""
+        import torch
+        rows = self.judge_rows if rows is None else np.asarray(rows)
+        V = self.geo.n_vox
+        dev = pe.device
+        exact = torch.empty((len(rows), V), dtype=torch.float32, device=dev)
+        self.exact_rows(pe, rows, exact)
+        sur = torch.empty_like(exact)
+        near = torch.empty_like(exact)
+        n_fb = 0
+        for i, r in enumerate(rows):
+            n_fb += self.apply(pe, int(r), 1, sur[i:i + 1], fallback=fallback)
+            self.near(pe, int(r), 1, near[i:i + 1])
+        torch.cuda.synchronize()
+        e = exact.double()
+        s = sur.double()
+        tot = float(e.sum().item())
+        big = e > 1e-3 * float(e.max().item())
+        rel = ((s - e).abs() / e.clamp_min(1e-300))[big]
+        out = {
+            "rows": len(rows),
+            "total_rel_err": float((s.sum().item() - tot) / tot) if tot > 0 else 0.0,
+            "l1_rel_err": float((s - e).abs().sum().item() / tot) if tot > 0 else 0.0,
+            "median_rel_err": float(rel.median().item()) if rel.numel() else 0.0,
+            "p90_rel_err": float(torch.quantile(rel.float(), 0.9).item()) if rel.numel() else 0.0,
+            "near_only_total_rel_err": float((near.double().sum().item() - tot) / tot) if tot > 0 else 0.0,
+            "near_only_l1_rel_err": float((near.double() - e).abs().sum().item() / tot) if tot > 0 else 0.0,
+            "fallback_tiles": n_fb,
+            "tiles": len(rows) * ((V + 63) // 64),
+        }
+        return out
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self.L.sm_surrogate_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

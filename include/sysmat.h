@@ -126,6 +126,16 @@ typedef struct SmScatterPlan SmScatterPlan;
 int sm_scatter_plan_create(const float* col, const float* det, const float* img, const float* phy, int rotation,
                            int collimator_variant, const unsigned* crystal_words, const unsigned* collimator_words,
                            int device, SmScatterPlan** plan);
+/* flags: without a bitmap the pair preparation evaluates the same attenuator
+ * criterion inline (identical results; no O(N^3)-bit table, e.g. 275 GB at N = 16384). */
+#define SM_PLAN_NO_CRYSTAL_BITMAP 1
+#define SM_PLAN_NO_COLLIMATOR_BITMAP 2
+int sm_scatter_plan_create_ex(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                              int collimator_variant, int flags, const unsigned* crystal_words,
+                              const unsigned* collimator_words, int device, SmScatterPlan** plan);
+/* Crystal scatter of the listed detector rows into dst_dev rows 0..n_rows-1. */
+int sm_scatter_plan_crystal_rows(SmScatterPlan* plan, const float* pe_dev, const int* rows, int n_rows,
+                                 float* dst_dev, long long row_stride, void* stream);
 int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
                             long long row_stride, void* stream);
 int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
@@ -191,6 +201,44 @@ int sm_mlp_pack(const float* params_dev, void* packed_dev, void* stream);
 /* Register-resident inference of n rows for n_members consecutive packed
  * members: y_dev[m*n + i]. */
 int sm_mlp_infer(const void* packed_dev, int n_members, const float* x_dev, long long n, float* y_dev, void* stream);
+
+/* ---- crystal-scatter surrogate (no counterpart in the reference) ----------
+ * Exact near field + MLP far field.  Per detector the n_near + n_ring sites with
+ * the largest single-site weight at the image centre are ranked once; the
+ * n_near best are summed exactly for every voxel (S_near), the MLP predicts the
+ * rest from 12 features (one uses the fixed-weight ring of the next n_ring
+ * sites), ensemble members that disagree send their 64-voxel tile to the exact
+ * kernel.  pe_dev: the full N x V PE matrix of the rotation (row stride V). */
+typedef struct SmSurrogate SmSurrogate;
+int sm_surrogate_create(const float* det, const float* img, const float* phy, int rotation,
+                        const unsigned* crystal_words, int use_bitmap, int n_near, int n_ring, int device,
+                        SmSurrogate** out);
+/* Feature / target normalisation: PE scale, scatter scale, target mean and
+ * deviation of log(S_far + phi), phi. */
+int sm_surrogate_set_norm(SmSurrogate* s, float pe_scale, float s_scale, float mu_y, float sig_y, float phi);
+/* Ranked sites of one row (site index, weight); returns the count. */
+int sm_surrogate_row_sites(const SmSurrogate* s, int row, int* sites, float* weights, int cap);
+/* S_near of rows [det_begin, +det_count) into dst_dev (asynchronous). */
+int sm_surrogate_near(SmSurrogate* s, const float* pe_dev, int det_begin, int det_count, float* dst_dev,
+                      long long row_stride, void* stream);
+/* Exact crystal scatter of listed rows into dst_dev rows 0..n_rows-1 (synchronous). */
+int sm_surrogate_exact_rows(SmSurrogate* s, const float* pe_dev, const int* rows, int n_rows, float* dst_dev,
+                            long long row_stride, void* stream);
+/* 12 features per (row, voxel): rows [det_begin, +det_count), voxels vox_dev[0..n_vox)
+ * (NULL: 0..n_vox-1), S_near of those rows in near_dev; x_dev[(row*n_vox + j)*12 + f]. */
+int sm_surrogate_features(SmSurrogate* s, const float* pe_dev, const float* near_dev, long long near_stride,
+                          int det_begin, int det_count, const int* vox_dev, int n_vox, float* x_dev, void* stream);
+/* dst_dev holds S_near on entry and S_near + predicted far field on exit;
+ * tiles where members differ by more than tol x value (value > floor_abs) are
+ * marked in tile_flags_dev (det_count x ceil(V/64) bytes, zeroed by the caller). */
+int sm_surrogate_apply(SmSurrogate* s, const void* packed_dev, int n_members, const float* pe_dev, int det_begin,
+                       int det_count, float* dst_dev, long long row_stride, float tol, float floor_abs,
+                       unsigned char* tile_flags_dev, void* stream);
+/* Exact recomputation of the marked tiles; returns the number of tiles. */
+long long sm_surrogate_fallback(SmSurrogate* s, const float* pe_dev, int det_begin, int det_count,
+                                const unsigned char* tile_flags_dev, float* dst_dev, long long row_stride,
+                                void* stream);
+int sm_surrogate_destroy(SmSurrogate* s);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,100 @@
+"""Crystal-scatter surrogate on the GPU.
+
+Primitives are exact: the bitmap-free pair preparation equals the bitmap one
+bit for bit; row-list exact scatter equals the contiguous launch; the near
+field summed over every site equals the exact scatter (FP64 order only).
+The learned far field is judged on held-out rows against the exact kernel,
+and the ablation compares it with the same pipeline without the network."""
+import numpy as np
+import pytest
+
+import oracle
+from sysmat import params
+from sysmat import scatter as sc
+
+pytestmark = pytest.mark.gpu
+
+
+def test_bitmap_free_prep_is_identical(lib):
+    import torch
+    g = params.tiny(seed=2, n_det_side=5, n_layers=3, image=(5, 4, 3), holes_side=3)
+    pe = torch.from_numpy(oracle.pe(g, sub=(2, 2, 2), precision="f64")).cuda()
+    N, V = g.n_det, g.n_vox
+    outs = []
+    for bm in (True, False):
+        p = sc.ScatterPlan(g, bitmaps=bm)
+        c = torch.zeros((N, V), dtype=torch.float32, device="cuda")
+        h = torch.zeros_like(c)
+        p.crystal(pe, 0, N, c)
+        p.collimator(0, N, h)
+        torch.cuda.synchronize()
+        outs.append((c.cpu().numpy(), h.cpu().numpy()))
+        p.close()
+    assert np.array_equal(outs[0][0], outs[1][0]) and np.array_equal(outs[0][1], outs[1][1])
+    assert outs[0][0].sum() > 0 and outs[0][1].sum() > 0
+
+
+def test_row_list_exact_equals_contiguous(lib):
+    import torch
+    g = params.tiny(seed=3, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3)
+    pe = torch.from_numpy(oracle.pe(g, sub=(2, 2, 2), precision="f64")).cuda()
+    N, V = g.n_det, g.n_vox
+    p = sc.ScatterPlan(g)
+    full = torch.zeros((N, V), dtype=torch.float32, device="cuda")
+    p.crystal(pe, 0, N, full)
+    rows = np.array([31, 2, 17, 16, 5], np.int32)
+    part = torch.zeros((len(rows), V), dtype=torch.float32, device="cuda")
+    p.crystal_rows(pe, rows, part)
+    torch.cuda.synchronize()
+    assert np.array_equal(part.cpu().numpy(), full.cpu().numpy()[rows])
+
+
+def test_near_field_over_all_sites_equals_exact(lib):
+    import torch
+    from sysmat.surrogate import ScatterSurrogate
+    g = params.tiny(seed=4, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3)
+    N, V = g.n_det, g.n_vox
+    pe = torch.from_numpy(oracle.pe(g, sub=(2, 2, 2), precision="f64")).cuda()
+    s = ScatterSurrogate(g, n_near=N - 1, n_ring=0)
+    near = torch.zeros((N, V), dtype=torch.float32, device="cuda")
+    s.near(pe, 0, N, near)
+    ex = torch.zeros_like(near)
+    s.exact_rows(pe, np.arange(N), ex)
+    torch.cuda.synchronize()
+    a, b = near.cpu().numpy().astype(np.float64), ex.cpu().numpy().astype(np.float64)
+    assert b.sum() > 0
+    np.testing.assert_allclose(a, b, rtol=1e-6, atol=1e-12 * b.max())
+    # ranking: weights descending
+    sites, w = s.row_sites(N - 1)
+    assert len(sites) > 0 and np.all(np.diff(w) <= 0)
+
+
+def c3_small():
+    g = params.config3()
+    g.img[0:3] = 16
+    g.img[3:6] = 2.0
+    return g
+
+
+def test_surrogate_judge_and_ablation(lib):
+    """Held-out rows of a 16^3-voxel C3 geometry: total crystal-scatter sum within 1 %,
+    and the network beats the same pipeline without it (near field only, or near field +
+    least-squares-scaled ring prior)."""
+    import torch
+    from sysmat.pe import PePlan
+    from sysmat.surrogate import ScatterSurrogate, TrainConfig
+    g = c3_small()
+    N, V = g.n_det, g.n_vox
+    pe = torch.empty((N, V), dtype=torch.float32, device="cuda")
+    plan = PePlan(g, voxel_sub=1, sub=(2, 2, 2))
+    plan.launch(0, N, pe)
+    plan.sync()
+    s = ScatterSurrogate(g, n_near=8, n_ring=24)
+    cfg = TrainConfig(train_rows=96, judge_rows=32, vox_per_row=4096, members=2, steps=1200, batch=4096, seed=0)
+    t = s.train(pe, cfg, log=lambda m: print(m, flush=True))
+    print("train timings", t, flush=True)
+    j = s.judge(pe)
+    print("judge", j, flush=True)
+    assert abs(j["total_rel_err"]) <= 0.01
+    assert j["l1_rel_err"] < j["near_only_l1_rel_err"]
+    assert set(s.train_rows).isdisjoint(set(s.judge_rows))
