@@ -11,6 +11,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <vector>
 
 #include "../../include/sysmat.h"
 #include "common.hpp"
@@ -29,6 +30,13 @@ struct PePlanImpl {
     PeParams base{};
     hipStream_t last_stream = nullptr;
     int tile_fixed = 0;
+    struct Launch {
+        int det_begin, det_count;
+        float* dst;
+        long long row_stride;
+        hipStream_t stream;
+    };
+    std::vector<Launch> pending;  // launches since the last sync (re-run with smaller tiles on overflow)
 };
 
 static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, PeParams& p) {
@@ -158,6 +166,10 @@ int sm_pe_plan_create(const float* col, const float* det, const float* img, cons
     int tx, ty, tz;
     auto_tiles(*o, tx, ty, tz);
     set_tiles(P->base, tx, ty, tz);
+    if (hipMemset(P->status.ptr, 0, sizeof(int)) != hipSuccess) {
+        delete P;
+        return set_error(SM_E_DEVICE, "status reset failed");
+    }
     P->base.det_begin = o->det_begin;
     P->base.det_count = o->det_count > 0 ? o->det_count : P->T.n_det - o->det_begin;
     *plan = reinterpret_cast<SmPePlan*>(P);
@@ -180,6 +192,7 @@ int sm_pe_plan_launch(SmPePlan* plan, int det_begin, int det_count, float* dst_d
     hipStream_t s = (hipStream_t)stream;
     P->last_stream = s;
     HIP_TRY(launch_pe(p, P->cry.ptr, P->hol.ptr, P->pla.ptr, P->slb.ptr, dst_dev, P->status.ptr, s));
+    P->pending.push_back({det_begin, det_count, dst_dev, p.row_stride, s});
     return SM_OK;
 }
 
@@ -190,9 +203,24 @@ int sm_pe_plan_sync(SmPePlan* plan) {
     int rc = use_device(P->device);
     if (rc < 0) return rc;
     int st = 0;
+    for (const auto& l : P->pending) HIP_TRY(hipStreamSynchronize(l.stream));
     HIP_TRY(hipStreamSynchronize(P->last_stream));
     HIP_TRY(hipMemcpy(&st, P->status.ptr, sizeof(int), hipMemcpyDeviceToHost));
-    if (st & 1) return set_error(SM_E_OVERFLOW, "cull list overflow in a tile: use smaller tiles");
+    std::vector<PePlanImpl::Launch> todo;
+    todo.swap(P->pending);
+    if (st & 1) {
+        // some tile's cull lists overflowed: recompute those launches with smaller tiles
+        int tx = std::max(1, P->base.tx / 2), ty = std::max(1, P->base.ty / 2), tz = std::max(1, P->base.tz / 2);
+        for (const auto& l : todo) {
+            PeParams p = P->base;
+            p.det_begin = l.det_begin;
+            p.det_count = l.det_count;
+            p.row_stride = l.row_stride;
+            set_tiles(p, tx, ty, tz);
+            if ((rc = run_sync(*P, p, l.dst, l.stream)) < 0) return rc;
+        }
+    }
+    HIP_TRY(hipMemset(P->status.ptr, 0, sizeof(int)));
     return SM_OK;
 }
 

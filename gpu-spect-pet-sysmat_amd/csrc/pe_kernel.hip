@@ -22,6 +22,8 @@
 //     (hole chords and plate length from FP64 end points).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "pe_kernel.hpp"
 #include "trace.hpp"
 
@@ -565,11 +567,20 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
 hipError_t launch_pe(const PeParams& p, const Crystal* cry, const Hole* hol, const Plate* pla, const Slab* slb,
                      float* dst, int* status, hipStream_t stream) {
     const long long tiles = (long long)p.ntx * p.nty * p.ntz;
-    const long long blocks = tiles * p.det_count;
-    if (blocks <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pe_tile_kernel, dim3((unsigned)blocks), dim3(kThreads), 0, stream, p, cry, hol, pla, slb, dst,
-                       status);
-    return hipGetLastError();
+    if (tiles <= 0 || p.det_count <= 0) return hipSuccess;
+    // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
+    const long long max_blocks = (1LL << 32) / kThreads - 1;
+    const int chunk = (int)std::max(1LL, std::min<long long>(p.det_count, max_blocks / tiles));
+    for (int d0 = 0; d0 < p.det_count; d0 += chunk) {
+        PeParams q = p;
+        q.det_begin = p.det_begin + d0;
+        q.det_count = std::min(chunk, p.det_count - d0);
+        hipLaunchKernelGGL(pe_tile_kernel, dim3((unsigned)(tiles * q.det_count)), dim3(kThreads), 0, stream, q, cry,
+                           hol, pla, slb, dst + (long long)d0 * p.row_stride, status);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace sysmat

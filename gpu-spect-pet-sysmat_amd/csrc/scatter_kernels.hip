@@ -16,6 +16,8 @@
 // probability, Range_phi, attenuation) are FP32.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kn.hpp"
 #include "scatter.hpp"
 #include "trace.hpp"
@@ -548,16 +550,42 @@ hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Cryst
 
 hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pairs, const int* counts,
                           const float* pe, float* dst, hipStream_t stream) {
-    long long grid;
+    const long long max_blocks = (1LL << 32) / 256 - 8;  // gridDim.x * blockDim.x < 2^32
     if (p.items) {
-        if (p.n_items <= 0) return hipSuccess;
-        grid = (p.n_items + 3) / 4;
-    } else {
-        if (p.n_blocks <= 0) return hipSuccess;
-        grid = (p.n_blocks + 7) / 8 * 8;
+        for (long long i0 = 0; i0 < p.n_items; i0 += max_blocks * 4) {
+            ScParams q = p;
+            q.items = p.items + i0;
+            q.n_items = std::min(p.n_items - i0, max_blocks * 4);
+            hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((q.n_items + 3) / 4)), dim3(256), 0, stream, q, rows,
+                               pairs, counts, pe, dst);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+        return hipSuccess;
     }
-    hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)grid), dim3(256), 0, stream, p, rows, pairs, counts, pe, dst);
-    return hipGetLastError();
+    if (p.n_blocks <= 0) return hipSuccess;
+    // detector chunks (multiples of 4 rows) small enough for one grid
+    const long long rg_max = std::max(1LL, max_blocks / std::max(1, p.n_tiles));
+    const int chunk = (int)std::min<long long>(p.det_count, rg_max * 4);
+    for (int d0 = 0; d0 < p.det_count; d0 += chunk) {
+        ScParams q = p;
+        q.det_count = std::min(chunk, p.det_count - d0);
+        if (p.row_list) {
+            q.row_list = p.row_list + d0;
+        } else {
+            q.det_begin = p.det_begin + d0;
+        }
+        q.n_rgroups = (q.det_count + 3) / 4;
+        q.n_blocks = (long long)q.n_tiles * q.n_rgroups;
+        const bool shift_dst = !(p.row_list && p.dst_by_row);
+        const long long grid = (q.n_blocks + 7) / 8 * 8;
+        hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)grid), dim3(256), 0, stream, q, rows,
+                           pairs + (long long)d0 * p.max_sites, counts + d0, pe,
+                           shift_dst ? dst + (long long)d0 * p.row_stride : dst);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_surrogate_rank(const ScParams& p, const SurrogateRank& k, const ScRow* rows, const Crystal* cry,

@@ -2,6 +2,8 @@
 // -> far-field prediction, written over the exact near field in place.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mlp_device.hpp"
 #include "surrogate.hpp"
 
@@ -141,11 +143,23 @@ hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const
 
 hipError_t launch_surrogate_apply(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                   const uint8_t* packed, const float* pe, float* dst, uint8_t* flags, hipStream_t s) {
-    const long long waves = (long long)q.det_count * ((q.V + 31) / 32);
-    if (waves <= 0) return hipSuccess;
-    hipLaunchKernelGGL(surrogate_apply_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, q, rows, keep,
-                       keep_cnt, packed, pe, dst, flags);
-    return hipGetLastError();
+    const long long per_row = (q.V + 31) / 32;  // waves per detector row
+    if (per_row <= 0 || q.det_count <= 0) return hipSuccess;
+    // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
+    const long long max_waves = ((1LL << 32) / 256 - 1) * 4;
+    const int chunk = (int)std::max(1LL, std::min<long long>(q.det_count, max_waves / per_row));
+    for (int d0 = 0; d0 < q.det_count; d0 += chunk) {
+        SgParams c = q;
+        c.det_begin = q.det_begin + d0;
+        c.det_count = std::min(chunk, q.det_count - d0);
+        const long long waves = (long long)c.det_count * per_row;
+        hipLaunchKernelGGL(surrogate_apply_kernel, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, s, c, rows, keep,
+                           keep_cnt, packed, pe, dst + (long long)d0 * q.row_stride,
+                           flags ? flags + (long long)d0 * q.n_tiles : nullptr);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 }  // namespace sysmat

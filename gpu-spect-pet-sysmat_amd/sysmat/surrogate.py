@@ -59,23 +59,23 @@ def usable_rows(geo: Geometry) -> np.ndarray:
 
 @dataclasses.dataclass
 class TrainConfig:
-    train_rows: int = 64           # exact rows used for training
-    judge_rows: int = 16           # disjoint held-out rows (the judge)
+    train_rows: int = 128          # exact rows used for training
+    judge_rows: int = 32           # disjoint held-out rows (the judge)
     vox_per_row: int = 8192        # sampled voxels per training row (all if V is smaller)
     members: int = 2
-    steps: int = 1500
+    steps: int = 3000
     batch: int = 4096
     lr: float = 2e-3
     seed: int = 0
     phi_rel: float = 1e-4          # log floor relative to the largest training element
-    tol: float = 0.02              # member disagreement that sends a tile to the exact kernel
+    tol: float = 0.1               # member disagreement that sends a tile to the exact kernel
     floor_rel: float = 1e-3        # disagreement is only checked above this fraction of the scale
 
 
 class ScatterSurrogate:
     """Holds the ranked site tables (HBM), the trained members and the normalisation."""
 
-    def __init__(self, geo: Geometry, rotation: int = 0, n_near: int = 8, n_ring: int = 24,
+    def __init__(self, geo: Geometry, rotation: int = 0, n_near: int = 32, n_ring: int = 224,
                  use_bitmap: Optional[bool] = None, device: int = 0):
         self.L = load()
         self.geo = geo
@@ -224,7 +224,7 @@ class ScatterSurrogate:
 
     # ---- inference ---------------------------------------------------------
     def apply(self, pe, det_begin: int, det_count: int, dst, row_stride: int = 0, fallback: bool = True,
-              n_members: Optional[int] = None, stream=None) -> int:
+              n_members: Optional[int] = None, tol: Optional[float] = None, stream=None) -> int:
         """Surrogate crystal scatter of rows [det_begin, +det_count) into dst; returns fallback tiles."""
         import torch
         if self.packed is None:
@@ -234,15 +234,17 @@ class ScatterSurrogate:
         n_tiles = (self.geo.n_vox + 63) // 64
         flags = torch.zeros(det_count * n_tiles, dtype=torch.uint8, device=pe.device) if fallback else None
         floor_abs = self.cfg.floor_rel * self.norm["s_scale"]
+        tol = self.cfg.tol if tol is None else tol
         check(self.L.sm_surrogate_apply(self._h, ptr(self.packed), nm, ptr(pe), det_begin, det_count, ptr(dst),
-                                        row_stride, ctypes.c_float(self.cfg.tol), ctypes.c_float(floor_abs),
+                                        row_stride, ctypes.c_float(tol), ctypes.c_float(floor_abs),
                                         None if flags is None else ptr(flags), _sh(stream)))
         if not fallback:
             return 0
         return int(check(self.L.sm_surrogate_fallback(self._h, ptr(pe), det_begin, det_count, ptr(flags), ptr(dst),
                                                       row_stride, _sh(stream))))
 
-    def judge(self, pe, rows: Optional[Sequence[int]] = None, fallback: bool = True) -> dict:
+    def judge(self, pe, rows: Optional[Sequence[int]] = None, fallback: bool = True,
+              tol: Optional[float] = None) -> dict:
         """Exact vs surrogate on held-out rows: total-sum error and per-element errors."""
         import torch
         rows = self.judge_rows if rows is None else np.asarray(rows)
@@ -254,7 +256,7 @@ class ScatterSurrogate:
         near = torch.empty_like(exact)
         n_fb = 0
         for i, r in enumerate(rows):
-            n_fb += self.apply(pe, int(r), 1, sur[i:i + 1], fallback=fallback)
+            n_fb += self.apply(pe, int(r), 1, sur[i:i + 1], fallback=fallback, tol=tol)
             self.near(pe, int(r), 1, near[i:i + 1])
         torch.cuda.synchronize()
         e = exact.double()
