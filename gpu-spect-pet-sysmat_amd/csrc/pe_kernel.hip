@@ -84,6 +84,25 @@ struct Att {
     int k;  // crystal index; negative (-(k+1)) when the crystal is rotated
 };
 
+// Fast form of the reference cull predicate |e/|e| - u| <= r/|e| + crit_self
+// (PESysMatGen.cu:506-589) with hardware rsqrt: +1 accept, -1 reject, 0 when
+// the two sides are closer than the error bound of this form (2e-6 absolute +
+// 1e-5 relative, against < 1.5e-6 from the approximations: unit-vector
+// components of magnitude <= 1 with a few ulp each), in which case the caller
+// evaluates the reference's operations.
+__device__ inline int cull_fast(float ex, float ey, float ez, float r, float ux, float uy, float uz, float crit_self) {
+    const float d2 = ex * ex + ey * ey + ez * ez;
+    const float id = __builtin_amdgcn_rsqf(d2);
+    const float vx = ex * id - ux, vy = ey * id - uy, vz = ez * id - uz;
+    const float l2 = vx * vx + vy * vy + vz * vz;
+    const float rhs = r * id + crit_self;
+    const float m = 1e-5f * rhs + 2e-6f;
+    const float lo = rhs - m, hi = rhs + m;
+    if (lo > 0.0f && l2 < lo * lo) return 1;
+    if (l2 > hi * hi) return -1;
+    return 0;
+}
+
 // Chord of the sub-ray through crystal k (reference order: PESysMatGen.cu:687-725).
 __device__ inline float crystal_chord(const WRay& R, const Crystal* __restrict__ cry, int k, float x1, float x2,
                                       float y1, float y2, float z1, float z2, bool rotated) {
@@ -160,16 +179,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
 
     // ---- workgroup cull: superset of every element's list ----------------
     // For any voxel centre I within rho of the tile centre I0, the reference
-    // predicate |u_k - u_d| <= r_k/D_k + r_d/D_d implies
-    // |u_k(I0) - u_d(I0)| <= r_k/(D_k0 - rho) + r_d/(D_d0 - rho) + 2 rho/D_k0 + 2 rho/D_d0
-    // (|a/|a| - b/|b|| <= 2|a - b|/|b|).  Ordered compaction keeps the lists
-    // sorted by index, so every element visits attenuators in index order.
+    // predicate |u_k - u_d| <= r_k/(2 D_k) + r_d/(2 D_d) implies
+    // |u_k(I0) - u_d(I0)| <= r_k/(2 (D_k0 - rho)) + r_d/(2 (D_d0 - rho))
+    //                        + 2 rho/(2 D_k0 - rho) + 2 rho/(2 D_d0 - rho)
+    // by the Dunkl-Williams inequality |a/|a| - b/|b|| <= 2 |a - b| / (|a| + |b|)
+    // with |a - b| <= rho, |a| >= D0 - rho, |b| = D0.  Ordered compaction keeps
+    // the lists sorted by index, so every element visits attenuators in index order.
     int nc = 0, nh = 0;
     {
         const float ddx = D.cx - tcx, ddy = D.cy - tcy, ddz = D.cz - tcz;
         const float Dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
         const float udx = ddx / Dd, udy = ddy / Dd, udz = ddz / Dd;
-        const float crit_d = (Dd > rho) ? 0.5f * D.rbound / (Dd - rho) + 2.0f * rho / Dd : 4.0f;
+        const float crit_d = (Dd > rho) ? 0.5f * D.rbound / (Dd - rho) + 2.0f * rho / (2.0f * Dd - rho) : 4.0f;
         for (int base = 0; base < p.n_det; base += kThreads) {
             const int k = base + tid;
             bool take = false;
@@ -182,7 +203,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                     take = true;
                 } else {
                     const float iD = 1.0f / Dk;
-                    const float bound = 0.5f * rb / (Dk - rho) + 2.0f * rho * iD + crit_d;
+                    const float bound = 0.5f * rb / (Dk - rho) + 2.0f * rho / (2.0f * Dk - rho) + crit_d;
                     const float vx = kx * iD - udx, vy = ky * iD - udy, vz = kz * iD - udz;
                     take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
                 }
@@ -212,7 +233,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                     take = true;
                 } else {
                     const float iD = 1.0f / Dh;
-                    const float bound = H.rbound / (Dh - rho) + 2.0f * rho * iD + crit_d;
+                    const float bound = H.rbound / (Dh - rho) + 2.0f * rho / (2.0f * Dh - rho) + crit_d;
                     const float vx = hx * iD - udx, vy = hy * iD - udy, vz = hz * iD - udz;
                     take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
                 }
@@ -297,6 +318,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
         // -- crystals: bundle overlap, then the exact predicate; survivors are
         //    kept as a bitmask and, when they fit, as compacted boxes
         int n_att = 0;
+        bool any_rotated = false;  // some listed attenuator needs its own frame
         for (int w = 0; w < ncw; w += 2) {
             const int j = w * 32 + lane;
             bool take = false;
@@ -313,16 +335,22 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                     take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
                 }
                 if (take) {
-                    float crit = K.rbound;
-                    const float dk =
-                        sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) + (zI - K.cz) * (zI - K.cz));
-                    crit = crit / dk / 2.0f;
-                    const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
-                    take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                           crit + crit_self;
+                    const int fast = cull_fast(K.cx - xI, K.cy - yI, K.cz - zI, 0.5f * K.rbound, ux, uy, uz, crit_self);
+                    if (fast != 0) {
+                        take = fast > 0;
+                    } else {  // near the boundary: the reference's operations
+                        float crit = K.rbound;
+                        const float dk =
+                            sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) + (zI - K.cz) * (zI - K.cz));
+                        crit = crit / dk / 2.0f;
+                        const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
+                        take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                               crit + crit_self;
+                    }
                 }
             }
             const unsigned long long bal = __ballot(take);
+            if (__ballot(take && K.axis == 0.0f)) any_rotated = true;
             if (take) {
                 const int slot = n_att + __popcll(bal & below);
                 if (slot < kListCap) {
@@ -359,12 +387,18 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                 }
                 if (take) {
                     const float yh = (H.y1 + H.y2) / 2.0f;
-                    float crit = H.rbound;
-                    const float dh = sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
-                    crit = crit / dh;
-                    const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
-                    take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                           crit + crit_self;
+                    const int fast = cull_fast(H.x - xI, yh - yI, H.z - zI, H.rbound, ux, uy, uz, crit_self);
+                    if (fast != 0) {
+                        take = fast > 0;
+                    } else {  // near the boundary: the reference's operations
+                        float crit = H.rbound;
+                        const float dh =
+                            sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
+                        crit = crit / dh;
+                        const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
+                        take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                               crit + crit_self;
+                    }
                 }
                 if (take) layer_holes |= 1u << (int)H.layer;
             }
@@ -518,7 +552,13 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                     }
                 }
                 // other crystals on the way (PESysMatGen.cu:687-725)
-                if (att_fast) {
+                if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
+                    // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the wave
+                    for (int j = 0; j < n_att; ++j) {
+                        const Att A = s_att[wave][j];
+                        att += chord_world(R, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
+                    }
+                } else if (att_fast) {
                     for (int j = 0; j < n_att; ++j) {
                         const Att A = s_att[wave][j];
                         const int k = uniform(A.k);
