@@ -1,19 +1,29 @@
-"""Benchmark: PE system-matrix elements/s on one MI355X.
+"""Benchmark: system-matrix elements/s on one MI355X.
 
-Workload: the realistic photoelectric system matrix (64^3 voxels of 0.5 mm, two
-5 mm tungsten plates with 8x8 holes each, 4 layers of 64x64 crystals =
-16384 detectors; 3^3 voxel sub-points x 2^3 crystal sub-cells = 216 sub-rays
-per element).  One step = one block of detector rows x all 262144 voxels,
-traced into an HBM-resident dense block; consecutive steps walk detector
-blocks spread over all four crystal layers.
+Headline (``value``): the realistic photoelectric system matrix -- 64^3 voxels
+of 0.5 mm, two 5 mm tungsten plates with 8x8 holes each, 4 layers of 64x64
+crystals (16384 detectors), 3^3 voxel sub-points x 2^3 crystal sub-cells per
+element, output kept as CSR (elements >= 1e-9 of the row maximum).  One step
+= one block of detector rows x all 262144 voxels: the PE tracer into an
+HBM-resident dense block, then the CSR count/scan/compaction kernels; the
+geometry is resident before timing starts.
 
-Prints one JSON line (metric, value = elements/s, roofline of the tracer
-kernel against the FP32 vector peak, CPU baseline of the FP32 restatement of
-the reference kernel on this host's cores).
+Extra objects on the same JSON line:
+  roofline      the tracer kernel against the FP32 vector peak (algorithmic
+                FLOPs per element from the op-counting restatement), HBM
+                traffic per launch from the committed rocprofv3 summary;
+  cpu_baseline  the FP32 restatement of the reference kernel (oracle/, g++
+                -O3 OpenMP) on this host's cores, bounded sample;
+  pe_scatter    PE + primary Compton scatter, plain exact kernels vs the
+                learned surrogate (exact near field + MLP far field +
+                exact fallback), on the collimator-scatter geometry (fully
+                measured) and on the surrogate geometry (N = 16384; exact
+                crystal scatter timed on a row sample and projected).
 """
 import argparse
 import json
 import os
+import re
 import sys
 import time
 
@@ -23,8 +33,9 @@ for p in (ROOT, PKG):
     if p not in sys.path:
         sys.path.insert(0, p)
 
-FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector/matrix peak (dense)
+FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (dense)
 HBM_PEAK_GBS = 8000.0
+METRIC = "sysmat elements/s (PE, PE+scatter) plain vs surrogate @1% err, 1 MI355X"
 
 
 def load_flops(key):
@@ -34,6 +45,20 @@ def load_flops(key):
             return json.load(f)[key]
     except (OSError, KeyError, ValueError):
         return None
+
+
+def load_traffic(name="pe_tile_kernel_c2.md"):
+    """HBM bytes per launch (FETCH + WRITE, calibrated) from the committed rocprofv3 summary."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", text)
+    r = re.search(r"rows_per_launch:\s*([0-9]+)", text)
+    if not m:
+        return None
+    return float(m.group(1)), int(r.group(1)) if r else None
 
 
 def cpu_baseline(geo, nv, sub, row0, n_rows, plane, n_planes=4):
@@ -54,6 +79,168 @@ def cpu_baseline(geo, nv, sub, row0, n_rows, plane, n_planes=4):
                       f"{threads} threads), {dt:.1f} s"}
 
 
+def sync_t():
+    import torch
+    torch.cuda.synchronize()
+    return time.perf_counter()
+
+
+def pe_full(geo, nv, sub, dev, block=2048):
+    import torch
+    from sysmat.pe import PePlan
+    N, V = geo.n_det, geo.n_vox
+    pe = torch.empty((N, V), dtype=torch.float32, device=dev)
+    plan = PePlan(geo, voxel_sub=nv, sub=sub)
+    t0 = sync_t()
+    for d0 in range(0, N, block):
+        plan.launch(d0, min(block, N - d0), pe[d0:])
+    plan.sync()
+    t = sync_t() - t0
+    plan.close()
+    return pe, t
+
+
+def scatter_c3(dev, log):
+    """Collimator-scatter geometry (2048 detectors, 32^3 voxels), everything measured."""
+    import torch
+    from sysmat import params
+    from sysmat.scatter import ScatterPlan
+    from sysmat.surrogate import ScatterSurrogate, TrainConfig
+    g = params.config3()
+    N, V = g.n_det, g.n_vox
+    pe, t_pe = pe_full(g, 1, (8, 16, 8), dev)
+    log(f"C3 PE {t_pe:.2f}s")
+    plan = ScatterPlan(g)
+    cry = torch.empty((N, V), dtype=torch.float32, device=dev)
+    col = torch.empty_like(cry)
+    t0 = sync_t()
+    plan.crystal(pe, 0, N, cry)
+    t_cry = sync_t() - t0
+    t0 = sync_t()
+    plan.collimator(0, N, col)
+    t_col = sync_t() - t0
+    log(f"C3 exact crystal {t_cry:.3f}s collimator {t_col:.3f}s")
+    sur = ScatterSurrogate(g)
+    t0 = sync_t()
+    tr = sur.train(pe, TrainConfig(seed=0))
+    t_train = sync_t() - t0
+    out = torch.empty_like(cry)
+    t0 = sync_t()
+    n_fb = sur.apply(pe, 0, N, out)
+    t_apply = sync_t() - t0
+    e, s = cry.double(), out.double()
+    full_exact = (pe.double().sum() + col.double().sum() + e.sum()).item()
+    tot_err = ((s.sum() - e.sum()) / e.sum()).item()
+    full_err = ((s.sum() - e.sum()) / full_exact)
+    l1 = ((s - e).abs().sum() / e.sum()).item()
+    judge = sur.judge(pe)
+    elems = N * V
+    plain_t = t_pe + t_cry + t_col
+    sur_t = t_pe + t_train + t_apply + t_col
+    res = {
+        "workload": "collimator-scatter geometry: 32^3 x 1 mm, 1 W plate 6x6 holes, 2 x 32x32 CdTe (2048 det), "
+                    "PE at reference sampling (1, 8x16x8)",
+        "elements": elems,
+        "plain": {"elements_per_s": round(elems / plain_t, 1), "pe_s": round(t_pe, 3),
+                  "crystal_scatter_s": round(t_cry, 3), "collimator_scatter_s": round(t_col, 3)},
+        "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
+                      "apply_s": round(t_apply, 3), "fallback_tiles": n_fb,
+                      "fallback_frac": round(n_fb / (N * ((V + 63) // 64)), 4),
+                      "train_pairs": tr["pairs"]},
+        "crystal_scatter_speedup_excl_training": round(t_cry / t_apply, 2),
+        "crystal_scatter_speedup_incl_training": round(t_cry / (t_apply + t_train), 2),
+        "crystal_total_sum_rel_err": round(tot_err, 6),
+        "system_matrix_total_sum_rel_err": round(float(full_err), 7),
+        "crystal_l1_rel_err": round(l1, 5),
+        "judge_heldout": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in judge.items()},
+    }
+    plan.close()
+    sur.close()
+    del pe, cry, col, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def scatter_c5(dev, log, sample_rows=128, apply_rows=512):
+    """Surrogate geometry (16384 detectors in 4 layers, image reduced to 32^3 x 1.6 mm):
+    PE and the collimator scatter in full; the exact crystal scatter and the surrogate
+    apply timed on seeded row samples and projected linearly to all rows."""
+    import numpy as np
+    import torch
+    from sysmat import params
+    from sysmat.scatter import ScatterPlan
+    from sysmat.surrogate import ScatterSurrogate, TrainConfig, usable_rows
+    g = params.config5()
+    g.img[0:3] = 32
+    g.img[3:6] = 1.6
+    N, V = g.n_det, g.n_vox
+    pe, t_pe = pe_full(g, 1, (2, 2, 2), dev)
+    log(f"C5r PE {t_pe:.2f}s")
+    plan = ScatterPlan(g, bitmaps=False)
+    col = torch.empty((N, V), dtype=torch.float32, device=dev)
+    t0 = sync_t()
+    plan.collimator(0, N, col)
+    t_col = sync_t() - t0
+    rng = np.random.default_rng(123)
+    rows = np.sort(rng.choice(usable_rows(g), sample_rows, replace=False))
+    ex = torch.empty((sample_rows, V), dtype=torch.float32, device=dev)
+    t0 = sync_t()
+    plan.crystal_rows(pe, rows, ex)
+    t_ex = sync_t() - t0
+    log(f"C5r exact crystal {sample_rows} rows {t_ex:.2f}s, collimator all rows {t_col:.2f}s")
+    sur = ScatterSurrogate(g, use_bitmap=False)
+    t0 = sync_t()
+    tr = sur.train(pe, TrainConfig(seed=0))
+    t_train = sync_t() - t0
+    judge = sur.judge(pe)
+    # surrogate apply (near field + MLP + fallback) on seeded rows, one call per row
+    arows = np.sort(rng.choice(np.arange(N), apply_rows, replace=False))
+    out = torch.empty((1, V), dtype=torch.float32, device=dev)
+    n_fb = 0
+    t0 = sync_t()
+    for r in arows:
+        n_fb += sur.apply(pe, int(r), 1, out)
+    t_ap = sync_t() - t0
+    per_row_exact = t_ex / sample_rows
+    per_row_sur = t_ap / apply_rows
+    t_cry_full = per_row_exact * N
+    t_sur_full = per_row_sur * N + t_train
+    elems = N * V
+    plain_t = t_pe + t_col + t_cry_full
+    sur_t = t_pe + t_col + t_sur_full
+    # total-sum error on the judge rows for the whole system matrix (PE + collimator + crystal)
+    jr = sur.judge_rows
+    jpe = pe[torch.as_tensor(jr, device=dev).long()].double().sum().item()
+    jcol = col[torch.as_tensor(jr, device=dev).long()].double().sum().item()
+    res = {
+        "workload": "surrogate geometry: 4 x 64x64 crystals (16384 det), 1 W plate 6x6 holes, image 32^3 x 1.6 mm, "
+                    "PE sampling (1, 2x2x2)",
+        "elements": elems,
+        "projection": f"exact crystal scatter timed on {sample_rows} seeded rows with a contribution, "
+                      f"surrogate apply (one call per row) on {apply_rows} seeded rows, both scaled to {N} rows; "
+                      f"PE, collimator scatter and training measured in full",
+        "plain": {"elements_per_s": round(elems / plain_t, 1), "pe_s": round(t_pe, 3),
+                  "collimator_scatter_s": round(t_col, 3), "crystal_scatter_s_projected": round(t_cry_full, 2)},
+        "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
+                      "apply_s_projected": round(per_row_sur * N, 3),
+                      "fallback_frac_sample": round(n_fb / (apply_rows * ((V + 63) // 64)), 4),
+                      "train_pairs": tr["pairs"]},
+        "crystal_scatter_speedup_incl_training": round(t_cry_full / t_sur_full, 2),
+        "pe_plus_scatter_speedup": round(plain_t / sur_t, 2),
+        "judge_heldout": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in judge.items()},
+        "system_matrix_total_sum_rel_err_judge_rows": None,
+    }
+    cry_tot = judge["total_rel_err"]
+    # crystal-scatter sum of the judge rows from the exact kernel
+    jex = torch.empty((len(jr), V), dtype=torch.float32, device=dev)
+    sur.exact_rows(pe, jr, jex)
+    jcry = jex.double().sum().item()
+    res["system_matrix_total_sum_rel_err_judge_rows"] = round(cry_tot * jcry / (jpe + jcol + jcry), 7)
+    plan.close()
+    sur.close()
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -61,18 +248,25 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--block", type=int, default=1024, help="detector rows per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-scatter", action="store_true", help="skip the PE+scatter legs")
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline rows (0: one per host core, <= 64)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3)")
+    ap.add_argument("--quiet", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build measures one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "this build measures one GPU; runs on several GPUs are not part of it",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+
+    def log(msg):
+        if not args.quiet:
+            print(msg, file=sys.stderr, flush=True)
 
     import numpy as np
     import torch
 
+    from sysmat import csr as csr_mod
     from sysmat import params
     from sysmat.pe import PePlan
 
@@ -90,41 +284,73 @@ def main():
     def block_of(i):
         return ((i * 5) % n_blocks) * B
 
-    for i in range(args.warmup):
+    def step(i, ev=None):
+        if ev is not None:
+            ev[0].record(stream)
         plan.launch(block_of(i), B, out, stream=stream)
+        if ev is not None:
+            ev[1].record(stream)
+        rp, col, val, rmax = csr_mod.csr_from_dense_device(out, 1e-9, stream=stream)
+        if ev is not None:
+            ev[2].record(stream)
+        return int(col.numel())
+
+    for i in range(args.warmup):
+        step(i)
     plan.sync()
     torch.cuda.synchronize()
 
-    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(args.steps)]
+    ev = [tuple(torch.cuda.Event(enable_timing=True) for _ in range(3)) for _ in range(args.steps)]
+    nnz = []
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        ev[i][0].record(stream)
-        plan.launch(block_of(args.warmup + i), B, out, stream=stream)
-        ev[i][1].record(stream)
+        nnz.append(step(args.warmup + i, ev[i]))
     plan.sync()
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    kernel_ms = [a.elapsed_time(b) for a, b in ev]
+    pe_ms = [a.elapsed_time(b) for a, b, _ in ev]
+    csr_ms = [b.elapsed_time(c) for _, b, c in ev]
     elems = args.steps * B * V
     value = elems / wall
     ms_per_step = wall * 1e3 / args.steps
-    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
-    nnz = int((out > 0).sum().item())
+    avg_kernel_s = sum(pe_ms) / len(pe_ms) / 1e3
+    log(f"C2 PE+CSR: {value:.3e} elements/s, PE {np.mean(pe_ms):.1f} ms, CSR {np.mean(csr_ms):.1f} ms per step")
 
     fpe = load_flops("c2_pe_nv3_sub222")
     roof = None
     if fpe:
         achieved = fpe * B * V / avg_kernel_s / 1e12
+        traffic = None
+        tr = load_traffic()
+        if tr and tr[1]:
+            traffic = round(tr[0] * B / tr[1])  # per launch of B rows
         roof = {"bound": "valu", "achieved": round(achieved, 3), "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": None,
-                "flops_per_element": fpe, "kernel": "pe_tile_kernel",
-                "avg_launch_ms": round(avg_kernel_s * 1e3, 3)}
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "flops_per_element": round(fpe, 1), "kernel": "pe_tile_kernel",
+                "avg_launch_ms": round(avg_kernel_s * 1e3, 3), "rows_per_launch": B,
+                "algorithmic_output_bytes_per_launch": B * V * 4}
 
     if args.calib_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
         b = torch.empty_like(a)
         b.copy_(a)
         torch.cuda.synchronize()
+
+    pe_scatter = None
+    if not args.no_scatter:
+        plan.close()
+        del out
+        torch.cuda.empty_cache()
+        pe_scatter = {}
+        try:
+            pe_scatter["c3"] = scatter_c3(dev, log)
+        except Exception as e:  # reported, not hidden
+            pe_scatter["c3"] = {"error": repr(e)}
+        try:
+            pe_scatter["c5"] = scatter_c5(dev, log)
+        except Exception as e:
+            pe_scatter["c5"] = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu:
@@ -135,7 +361,7 @@ def main():
         cpu = cpu_baseline(geo, nv, sub, row0, n_rows, plane)
 
     rec = {
-        "metric": "sysmat elements/s (PE, PE+scatter) plain vs surrogate @1% err, 1 MI355X",
+        "metric": METRIC,
         "value": round(value, 1),
         "unit": "elements/s",
         "n_gpus": 1,
@@ -147,16 +373,20 @@ def main():
         "vs_baseline": None,
         "dtype": "f32 (f64 accumulate)",
         "data": "synthetic geometry from seed 0",
-        "config": {"workload": "PE realistic: 64^3 x 0.5 mm image, 2 W plates 8x8 holes, 4x64x64 crystals, "
-                               "3^3 voxel x 2^3 crystal sub-points, dense HBM block",
-                   "detectors_per_step": B, "voxels": V, "nonzero_in_last_block": nnz},
+        "config": {"workload": "PE realistic: 64^3 x 0.5 mm image, 2 W plates 8x8 holes, 4x64x64 crystals "
+                               "(16384 det), 3^3 voxel x 2^3 crystal sub-points, CSR output >= 1e-9 row max",
+                   "detectors_per_step": B, "voxels": V, "csr_nnz_per_step": int(np.mean(nnz)),
+                   "pe_kernel_ms_per_step": round(float(np.mean(pe_ms)), 3),
+                   "csr_ms_per_step": round(float(np.mean(csr_ms)), 3)},
         "roofline": roof,
         "cpu_baseline": cpu,
+        "pe_scatter": pe_scatter,
     }
     if cpu:
         rec["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(rec))
-    plan.close()
+    if args.no_scatter:
+        plan.close()
 
 
 if __name__ == "__main__":
