@@ -896,3 +896,165 @@ int sm_surrogate_destroy(SmSurrogate* S) {
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Batched scatter over collimator variants (shared detector and image): every
+// variant's collimator term then crystal term onto the same rows, all kernels
+// on one stream with preallocated workspace, optionally captured once into a
+// hipGraph and replayed.
+// ---------------------------------------------------------------------------
+struct SmScatterBatch {
+    std::vector<std::unique_ptr<ScatterGeom>> G;
+    DeviceBuf<ScPair> pairs;
+    DeviceBuf<int> counts;
+    int block = 0;  // detector rows per prep block
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<const float*> cap_pe;
+    std::vector<float*> cap_dst;
+    hipStream_t cap_stream = nullptr;
+    ~SmScatterBatch() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        if (own_stream) (void)hipStreamDestroy(own_stream);
+    }
+};
+
+namespace {
+
+// Kernel launches only (capturable): for every variant and row block, collimator
+// prep + accumulate (write), crystal prep + accumulate (add).
+int batch_enqueue(SmScatterBatch& B, const float* const* pe, float* const* dst, hipStream_t s) {
+    for (size_t vi = 0; vi < B.G.size(); ++vi) {
+        ScatterGeom& G = *B.G[vi];
+        const int n = G.T.n_det;
+        for (int b0 = 0; b0 < n; b0 += B.block) {
+            const int cnt = std::min(B.block, n - b0);
+            for (int term = 0; term < 2; ++term) {  // 0 collimator (writes), 1 crystal (adds)
+                ScParams p = G.base;
+                p.mode = term == 0 ? 1 : 0;
+                p.max_sites = term == 0 ? std::max(1, G.n_holes0) : n;
+                p.det_begin = b0;
+                p.det_count = cnt;
+                p.row_stride = p.V;
+                p.n_tiles = (int)((p.V + 63) / 64);
+                p.n_rgroups = (cnt + 3) / 4;
+                p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
+                p.accumulate = term;
+                if (term == 0 && G.n_holes0 <= 0) {
+                    // no layer-0 holes: a zero collimator term; the crystal term then writes
+                    continue;
+                }
+                if (term == 1 && G.n_holes0 <= 0) p.accumulate = 0;
+                HIP_TRY(launch_scatter_prep(p, G.rows.ptr, G.cry.ptr, G.hol.ptr, nullptr, B.pairs.ptr, B.counts.ptr, s));
+                HIP_TRY(launch_scatter(p, G.rows.ptr, B.pairs.ptr, B.counts.ptr, pe[vi], dst[vi] + (long long)b0 * p.V, s));
+            }
+        }
+    }
+    return SM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int sm_scatter_batch_create(int n_variants, const float* const* cols, const float* det, const float* img,
+                            const float* phy, int rotation, int device, SmScatterBatch** out) {
+    clear_error();
+    if (n_variants < 1 || !cols || !det || !img || !phy || !out) return set_error(SM_E_ARG, "bad argument");
+    int rc = use_device(device);
+    if (rc < 0) return rc;
+    auto* B = new SmScatterBatch();
+    B->device = device;
+    auto fail = [&](int code) {
+        delete B;
+        return code;
+    };
+    long long max_sites = 1;
+    for (int v = 0; v < n_variants; ++v) {
+        if (!cols[v]) return fail(set_error(SM_E_ARG, "null collimator parameters"));
+        B->G.emplace_back(new ScatterGeom());
+        if ((rc = build_geom(cols[v], det, img, phy, device, rotation, *B->G.back())) < 0) return fail(rc);
+        if (B->G.back()->base.V != B->G[0]->base.V || B->G.back()->T.n_det != B->G[0]->T.n_det)
+            return fail(set_error(SM_E_ARG, "variants must share the detector and the image"));
+        max_sites = std::max<long long>(max_sites, std::max(B->G.back()->T.n_det, B->G.back()->n_holes0));
+    }
+    const long long pair_budget = 2048ll << 20;
+    B->block = (int)std::max(4ll, std::min<long long>(B->G[0]->T.n_det, pair_budget / (max_sites * (long long)sizeof(ScPair))));
+    B->block = (B->block + 3) / 4 * 4;
+    if ((rc = B->pairs.alloc((size_t)B->block * max_sites)) < 0) return fail(rc);
+    if ((rc = B->counts.alloc(B->block)) < 0) return fail(rc);
+    *out = B;
+    return SM_OK;
+}
+
+// dst_v = collimator scatter + crystal scatter of variant v (N x V floats each, device).
+// use_graph: the first call captures the launch sequence into a hipGraph (on
+// `stream`, or an internal stream when NULL) and every call replays it; a call
+// with other buffers re-captures.  Synchronous.
+int sm_scatter_batch_run(SmScatterBatch* B, const float* const* pe_devs, float* const* dst_devs, void* stream,
+                         int use_graph) {
+    clear_error();
+    if (!B || !pe_devs || !dst_devs) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(B->device);
+    if (rc < 0) return rc;
+    hipStream_t s = (hipStream_t)stream;
+    if (!use_graph) {
+        if ((rc = batch_enqueue(*B, pe_devs, dst_devs, s)) < 0) return rc;
+        HIP_TRY(hipStreamSynchronize(s));
+        return SM_OK;
+    }
+    if (!s) {
+        if (!B->own_stream) HIP_TRY(hipStreamCreateWithFlags(&B->own_stream, hipStreamNonBlocking));
+        s = B->own_stream;
+    }
+    const size_t nv = B->G.size();
+    bool same = B->exec && B->cap_stream == s && B->cap_pe.size() == nv;
+    for (size_t v = 0; same && v < nv; ++v) same = B->cap_pe[v] == pe_devs[v] && B->cap_dst[v] == dst_devs[v];
+    if (!same) {
+        if (B->exec) {
+            (void)hipGraphExecDestroy(B->exec);
+            B->exec = nullptr;
+        }
+        if (B->graph) {
+            (void)hipGraphDestroy(B->graph);
+            B->graph = nullptr;
+        }
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        rc = batch_enqueue(*B, pe_devs, dst_devs, s);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s, &g);
+        if (rc < 0) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        if (e != hipSuccess) return set_error(SM_E_DEVICE, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        B->graph = g;
+        HIP_TRY(hipGraphInstantiate(&B->exec, B->graph, nullptr, nullptr, 0));
+        B->cap_pe.assign(pe_devs, pe_devs + nv);
+        B->cap_dst.assign(dst_devs, dst_devs + nv);
+        B->cap_stream = s;
+    }
+    HIP_TRY(hipGraphLaunch(B->exec, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return SM_OK;
+}
+
+int sm_scatter_batch_graph_nodes(const SmScatterBatch* B) {
+    if (!B || !B->graph) return 0;
+    size_t n = 0;
+    if (hipGraphGetNodes(B->graph, nullptr, &n) != hipSuccess) return set_error(SM_E_DEVICE, "hipGraphGetNodes");
+    return (int)n;
+}
+
+int sm_scatter_batch_destroy(SmScatterBatch* B) {
+    if (B) {
+        use_device(B->device);
+        delete B;
+    }
+    return SM_OK;
+}
+
+}  // extern "C"

@@ -72,6 +72,7 @@ struct ScParams {
     int site_cap;             // accumulate over the first site_cap pairs of each row only
     const int2* items;        // list mode: wave w of block b handles items[4b + w] = (rr, voxel tile)
     long long n_items;
+    int accumulate;           // 1: dst += element (e.g. crystal scatter onto the collimator term)
 };
 
 // Scatter-surrogate near field: per detector the n_near + n_ring sites with the

@@ -460,7 +460,11 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
             acc += site_term(p, R, S, q, kc, w);
         }
     }
-    if (active) dst[drow * p.row_stride + v] = (float)acc;
+    if (active) {
+        float* o = dst + drow * p.row_stride + v;
+        if (p.accumulate) *o += (float)acc;
+        else *o = (float)acc;
+    }
 }
 
 // Scatter-surrogate ranking: one workgroup per detector row; every candidate

@@ -87,6 +87,11 @@ SIGNATURES = {
     "sm_surrogate_fallback": (ctypes.c_longlong, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, _VP, ctypes.c_longlong,
                                                   _VP]),
     "sm_surrogate_destroy": (ctypes.c_int, [_VP]),
+    "sm_scatter_batch_create": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(_VP), _VP, _VP, _VP, ctypes.c_int,
+                                               ctypes.c_int, ctypes.POINTER(_VP)]),
+    "sm_scatter_batch_run": (ctypes.c_int, [_VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP), _VP, ctypes.c_int]),
+    "sm_scatter_batch_graph_nodes": (ctypes.c_int, [_VP]),
+    "sm_scatter_batch_destroy": (ctypes.c_int, [_VP]),
     "sm_csr_count_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,
                                            _VP, _VP, ctypes.POINTER(ctypes.c_longlong), _VP]),
     "sm_csr_fill_device": (ctypes.c_int, [_VP, ctypes.c_int, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_double,

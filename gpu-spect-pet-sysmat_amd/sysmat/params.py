@@ -260,16 +260,20 @@ def config3(seed: int = 0) -> Geometry:
     return build(layers, holes, cry, dict(shape=(32, 32, 32), voxel=(1.0, 1.0, 1.0), fov2col=50.0), PHYS_140)
 
 
-def config4_variants(seed: int = 0, n_variants: int = 6) -> List[Geometry]:
+def config4_variants(seed: int = 0, n_variants: int = 6, image=(48, 48, 48), voxel: float = 0.75,
+                     det_side: int = 48, n_layers: int = 3) -> List[Geometry]:
     """Crystal + collimator scatter over collimator variants drawn from ``seed``:
     1-3 plates, hole diameter U(0.5, 2.0), 4x4..12x12 grids; shared 48^3 x 0.75 mm
-    image and 3 layers of 48x48 crystals (1.2 mm pitch, 2 mm thick, mu_c 0.07)."""
+    image and shared detector, 3 layers of 48x48 crystals (1.2 mm pitch, 2 mm thick,
+    mu_c 0.07), placed 10 mm behind the deepest (3-plate) collimator so every
+    variant sees the same detector."""
     rng = np.random.default_rng(seed)
     out = []
     t = 4.0
+    gap = 12.0
+    max_plates = 3
     for _ in range(n_variants):
-        n_pl = int(rng.integers(1, 4))
-        gap = 12.0
+        n_pl = int(rng.integers(1, max_plates + 1))
         offs = [i * gap for i in range(n_pl)]
         layers = [dict(width=72.0, thickness=t, height=72.0, offset=o, mut=W_MU[0], mupe=W_MU[1], muc=W_MU[2])
                   for o in offs]
@@ -277,8 +281,8 @@ def config4_variants(seed: int = 0, n_variants: int = 6) -> List[Geometry]:
         d = float(rng.uniform(0.5, 2.0))
         pitch = max(2.0 * d, 60.0 / g)
         holes = np.concatenate([hole_grid(g, pitch, d, o - t / 2, o + t / 2, m) for m, o in enumerate(offs)])
-        cry = crystal_layers(48, 1.2, 2.0, offs[-1] + t / 2 + 10.0, 3)
-        out.append(build(layers, holes, cry, dict(shape=(48, 48, 48), voxel=(0.75, 0.75, 0.75), fov2col=50.0),
+        cry = crystal_layers(det_side, 1.2, 2.0, (max_plates - 1) * gap + t / 2 + 10.0, n_layers)
+        out.append(build(layers, holes, cry, dict(shape=tuple(image), voxel=(voxel,) * 3, fov2col=50.0),
                          PHYS_140))
     return out
 

@@ -11,7 +11,7 @@ file, scatter.cu:2145-2157).
 from __future__ import annotations
 
 import ctypes
-from typing import Optional, Tuple
+from typing import Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -132,6 +132,45 @@ class ScatterPlan:
     def close(self) -> None:
         if getattr(self, "_h", None):
             self._lib.sm_scatter_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class ScatterBatch:
+    """Crystal + collimator scatter of several collimator variants (shared detector and image),
+    all launches on one stream, optionally captured once into a hipGraph and replayed."""
+
+    def __init__(self, variants: Sequence[Geometry], rotation: int = 0, device: int = 0):
+        self._lib = load()
+        g0 = variants[0]
+        self._cols = [_f32(g.col) for g in variants]
+        self._keep = [_f32(g0.det), _img(g0.img), _f32(g0.phy)]
+        arr = (ctypes.c_void_p * len(variants))(*[c.ctypes.data for c in self._cols])
+        h = ctypes.c_void_p()
+        check(self._lib.sm_scatter_batch_create(len(variants), arr, ptr(self._keep[0]), ptr(self._keep[1]),
+                                                ptr(self._keep[2]), rotation, device, ctypes.byref(h)))
+        self._h = h
+        self.n = len(variants)
+
+    def run(self, pe_devs, dst_devs, use_graph: bool = True, stream=None) -> None:
+        pe = (ctypes.c_void_p * self.n)(*[ptr(t) for t in pe_devs])
+        dst = (ctypes.c_void_p * self.n)(*[ptr(t) for t in dst_devs])
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        check(self._lib.sm_scatter_batch_run(self._h, pe, dst, ctypes.c_void_p(s) if s else None,
+                                             1 if use_graph else 0))
+
+    @property
+    def graph_nodes(self) -> int:
+        return int(self._lib.sm_scatter_batch_graph_nodes(self._h))
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            self._lib.sm_scatter_batch_destroy(self._h)
             self._h = None
 
     def __del__(self):

@@ -202,6 +202,24 @@ int sm_mlp_pack(const float* params_dev, void* packed_dev, void* stream);
  * members: y_dev[m*n + i]. */
 int sm_mlp_infer(const void* packed_dev, int n_members, const float* x_dev, long long n, float* y_dev, void* stream);
 
+/* ---- batched scatter over collimator variants (no counterpart in the reference,
+ * which handles one geometry per process) ------------------------------------
+ * Variants share the detector, image and physics; cols[v] are their collimator
+ * parameter arrays.  run: dst_devs[v] = collimator term (combined model,
+ * scatter.cu:1544-1936) + crystal term (scatter.cu:1077-1437) of variant v for
+ * the N x V PE matrix pe_devs[v], attenuator criterion evaluated inline.  With
+ * use_graph != 0 the launch sequence is captured once into a hipGraph (on
+ * `stream`, or an internal non-blocking stream when NULL) and replayed;
+ * synchronous. */
+typedef struct SmScatterBatch SmScatterBatch;
+int sm_scatter_batch_create(int n_variants, const float* const* cols, const float* det, const float* img,
+                            const float* phy, int rotation, int device, SmScatterBatch** out);
+int sm_scatter_batch_run(SmScatterBatch* b, const float* const* pe_devs, float* const* dst_devs, void* stream,
+                         int use_graph);
+/* Nodes of the captured graph (0 before the first graph run). */
+int sm_scatter_batch_graph_nodes(const SmScatterBatch* b);
+int sm_scatter_batch_destroy(SmScatterBatch* b);
+
 /* ---- crystal-scatter surrogate (no counterpart in the reference) ----------
  * Exact near field + MLP far field.  Per detector the n_near + n_ring sites with
  * the largest single-site weight at the image centre are ranked once; the
