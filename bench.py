@@ -253,6 +253,9 @@ def main():
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3)")
     ap.add_argument("--quiet", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true", help="use the default 4x4x4 voxel tile")
+    ap.add_argument("--tune-cache", default=os.environ.get("SYSMAT_TUNE_CACHE",
+                                                            os.path.join(ROOT, ".tile_cache.txt")))
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; runs on several GPUs are not part of it",
@@ -268,7 +271,7 @@ def main():
 
     from sysmat import csr as csr_mod
     from sysmat import params
-    from sysmat.pe import PePlan
+    from sysmat.pe import PePlan, autotune
 
     geo = params.config2(seed=0)
     nv, sub = 3, (2, 2, 2)
@@ -277,7 +280,15 @@ def main():
     n_blocks = N // B
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
-    plan = PePlan(geo, voxel_sub=nv, sub=sub, device=0)
+    tile, tune_info = (4, 4, 4), None
+    if not args.no_autotune:
+        os.makedirs(os.path.dirname(args.tune_cache) or ".", exist_ok=True)
+        t0 = time.perf_counter()
+        tile, ns_el, cached = autotune(geo, sample_rows=64, cache_path=args.tune_cache, voxel_sub=nv, sub=sub)
+        tune_info = {"tile": list(tile), "sample_ns_per_element": round(ns_el, 3), "cached": cached,
+                     "tuning_s": round(time.perf_counter() - t0, 2)}
+        log(f"autotune: tile {tile} ({ns_el:.2f} ns/element on the sample, cached={cached})")
+    plan = PePlan(geo, voxel_sub=nv, sub=sub, tile=tile, device=0)
     out = torch.empty((B, V), dtype=torch.float32, device=dev)
     stream = torch.cuda.current_stream()
 
@@ -377,6 +388,7 @@ def main():
                                "(16384 det), 3^3 voxel x 2^3 crystal sub-points, CSR output >= 1e-9 row max",
                    "detectors_per_step": B, "voxels": V, "csr_nnz_per_step": int(np.mean(nnz)),
                    "pe_kernel_ms_per_step": round(float(np.mean(pe_ms)), 3),
+                   "voxel_tile": list(tile), "autotune": tune_info,
                    "csr_ms_per_step": round(float(np.mean(csr_ms)), 3)},
         "roofline": roof,
         "cpu_baseline": cpu,

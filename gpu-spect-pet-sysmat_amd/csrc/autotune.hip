@@ -1,0 +1,158 @@
+// Tile-shape autotuner of the PE tracer: times candidate voxel tiles on a
+// sample of detector rows spread over the detector and keeps the fastest.
+// Results persist in a small text cache keyed by a hash of the geometry, the
+// image, the sampling and the GPU architecture, so a configuration is tuned
+// once per machine type.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/sysmat.h"
+#include "common.hpp"
+
+using namespace sysmat;
+
+namespace {
+
+uint64_t fnv1a(const void* data, size_t n, uint64_t h) {
+    const unsigned char* p = static_cast<const unsigned char*>(data);
+    for (size_t i = 0; i < n; ++i) {
+        h ^= p[i];
+        h *= 0x100000001B3ull;
+    }
+    return h;
+}
+
+std::string tune_key(const float* col, const float* det, const float* img, const SmPeOptions& o, const char* arch) {
+    uint64_t h = 0xcbf29ce484222325ull;
+    const int nl = (int)(col[0] + 0.000001f);
+    int nh = 0;
+    for (int m = 0; m < nl && m < 9; ++m) nh += (int)(col[(m + 1) * 10] + 0.000001f);
+    h = fnv1a(col, sizeof(float) * (size_t)(100 + 9 * nh), h);
+    const int n = (int)(det[0] + 0.000001f);
+    h = fnv1a(det, sizeof(float) * (size_t)(1 + 12 * n), h);
+    h = fnv1a(img, sizeof(float) * 12, h);
+    const int samp[4] = {o.voxel_sub, o.sub_x, o.sub_y, o.sub_z};
+    h = fnv1a(samp, sizeof(samp), h);
+    h = fnv1a(arch, std::strlen(arch), h);
+    char buf[64];
+    std::snprintf(buf, sizeof(buf), "%016llx", (unsigned long long)h);
+    return buf;
+}
+
+bool cache_lookup(const char* path, const std::string& key, int t[3], double& ns) {
+    if (!path) return false;
+    FILE* f = std::fopen(path, "r");
+    if (!f) return false;
+    char k[64];
+    int a, b, c;
+    double v;
+    bool found = false;
+    while (std::fscanf(f, "%63s %d %d %d %lf", k, &a, &b, &c, &v) == 5) {
+        if (key == k) {
+            t[0] = a; t[1] = b; t[2] = c;
+            ns = v;
+            found = true;  // the last entry for a key wins
+        }
+    }
+    std::fclose(f);
+    return found;
+}
+
+void cache_store(const char* path, const std::string& key, const int t[3], double ns) {
+    if (!path) return;
+    FILE* f = std::fopen(path, "a");
+    if (!f) return;
+    std::fprintf(f, "%s %d %d %d %.6f\n", key.c_str(), t[0], t[1], t[2], ns);
+    std::fclose(f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int sm_pe_autotune(const float* col, const float* det, const float* img, const SmPeOptions* base, int sample_rows,
+                   const char* cache_path, SmPeOptions* best, double* best_ns_per_element) {
+    clear_error();
+    if (!col || !det || !img || !base || !best) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(base->device);
+    if (rc < 0) return rc;
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, base->device));
+    const std::string key = tune_key(col, det, img, *base, prop.gcnArchName);
+    *best = *base;
+    int t[3];
+    double ns = 0.0;
+    if (cache_lookup(cache_path, key, t, ns)) {
+        best->tile_x = t[0];
+        best->tile_y = t[1];
+        best->tile_z = t[2];
+        if (best_ns_per_element) *best_ns_per_element = ns;
+        return 1;  // from the cache
+    }
+    const int N = (int)std::floor(det[0] + 0.001f);
+    const int X = (int)std::floor(img[0] + 0.001f), Y = (int)std::floor(img[1] + 0.001f);
+    const int Z = (int)std::floor(img[2] + 0.001f);
+    const int zc = base->z_count > 0 ? base->z_count : Z - base->z_begin;
+    const long long V = (long long)X * Y * zc;
+    if (N <= 0 || V <= 0) return set_error(SM_E_ARG, "empty geometry");
+    // sample: 4 blocks of rows spread over the detector (layers differ in cost)
+    const int per = std::max(1, std::min(sample_rows > 0 ? sample_rows : 64, N) / 4);
+    std::vector<int> starts;
+    for (int b = 0; b < 4; ++b) starts.push_back(std::min(N - per, (int)((long long)(2 * b + 1) * N / 8)));
+    DeviceBuf<float> dst;
+    if ((rc = dst.alloc((size_t)per * V)) < 0) return rc;
+    static const int cand[][3] = {{4, 4, 4}, {8, 4, 2}, {4, 8, 2}, {2, 4, 8}, {8, 8, 1}, {4, 4, 2},
+                                  {2, 2, 4}, {8, 8, 2}, {4, 4, 8}, {2, 8, 4}, {8, 2, 4}, {16, 4, 1}};
+    hipEvent_t e0, e1;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    double best_ms = -1.0;
+    int bt[3] = {4, 4, 4};
+    for (const auto& c : cand) {
+        if (c[0] > X || c[1] > Y || c[2] > zc) continue;
+        SmPeOptions o = *base;
+        o.tile_x = c[0];
+        o.tile_y = c[1];
+        o.tile_z = c[2];
+        o.det_begin = 0;
+        o.det_count = N;
+        SmPePlan* plan = nullptr;
+        if ((rc = sm_pe_plan_create(col, det, img, &o, &plan)) < 0) break;
+        // warm-up on the first block, then the timed sample
+        rc = sm_pe_plan_launch(plan, starts[0], per, dst.ptr, V, nullptr);
+        if (rc >= 0) rc = sm_pe_plan_sync(plan);
+        (void)hipEventRecord(e0, nullptr);
+        for (int s : starts)
+            if (rc >= 0) rc = sm_pe_plan_launch(plan, s, per, dst.ptr, V, nullptr);
+        (void)hipEventRecord(e1, nullptr);
+        if (rc >= 0) rc = sm_pe_plan_sync(plan);
+        float ms = 0.0f;
+        if (rc >= 0) (void)hipEventElapsedTime(&ms, e0, e1);
+        sm_pe_plan_destroy(plan);
+        if (rc < 0) break;
+        if (best_ms < 0.0 || ms < best_ms) {
+            best_ms = ms;
+            bt[0] = c[0]; bt[1] = c[1]; bt[2] = c[2];
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc < 0) return rc;
+    if (best_ms < 0.0) return set_error(SM_E_ARG, "no tile candidate fits the image");
+    best->tile_x = bt[0];
+    best->tile_y = bt[1];
+    best->tile_z = bt[2];
+    ns = best_ms * 1e6 / ((double)per * 4.0 * (double)V);
+    if (best_ns_per_element) *best_ns_per_element = ns;
+    cache_store(cache_path, key, bt, ns);
+    return 0;
+}
+
+}  // extern "C"

@@ -6,7 +6,8 @@
 // [rotation][detector][z][y][x]).
 // Extensions (not in the reference): -sub NV CX CY CZ sets the sampling
 // (voxel sub-points per axis, crystal sub-cells; default 1 8 16 8 = reference),
-// -block N the detector rows per GPU launch, -csr REL writes
+// -block N the detector rows per GPU launch, -tune picks the voxel tile with the
+// autotuner (cached in $HOME/.cache/sysmat_tiles.txt), -csr REL writes
 // PE_SysMat_shift_<x>_<y>_<z>.csr instead (rows = rotation-major detectors,
 // elements >= REL x row max kept; the .csr layout is given in sysmat.h).
 #include <algorithm>
@@ -22,6 +23,7 @@ int main(int argc, char* argv[]) {
     sm_pe_default_options(&o);
     int block = 0;
     double csr_rel = -1.0;
+    bool tune = false;
     for (int i = 1; i < argc; ++i) {
         if (std::strcmp(argv[i], "-cuda") == 0 && i + 1 < argc) {
             o.device = std::atoi(argv[++i]);
@@ -32,11 +34,13 @@ int main(int argc, char* argv[]) {
             o.sub_z = std::atoi(argv[++i]);
         } else if (std::strcmp(argv[i], "-block") == 0 && i + 1 < argc) {
             block = std::atoi(argv[++i]);
+        } else if (std::strcmp(argv[i], "-tune") == 0) {
+            tune = true;
         } else if (std::strcmp(argv[i], "-csr") == 0 && i + 1 < argc) {
             csr_rel = std::atof(argv[++i]);
             if (!(csr_rel >= 0.0 && csr_rel <= 1.0)) drv::fail("-csr threshold must be in [0, 1]", 2);
         } else if (std::strcmp(argv[i], "-h") == 0 || std::strcmp(argv[i], "--help") == 0) {
-            std::printf("Usage: %s [-cuda GPU_ID] [-sub NV CX CY CZ] [-block ROWS] [-csr REL]\n", argv[0]);
+            std::printf("Usage: %s [-cuda GPU_ID] [-sub NV CX CY CZ] [-block ROWS] [-tune] [-csr REL]\n", argv[0]);
             return 0;
         } else {
             std::fprintf(stderr, "Unknown parameter or missing argument: %s\n", argv[i]);
@@ -55,6 +59,20 @@ int main(int argc, char* argv[]) {
     std::printf("numDet=%d  fullVox=%lld  rotations=%d  sysmatBytes=%lld (%.2f GB)\n", N, V, R, total * 4,
                 (double)total * 4 / (1024.0 * 1024.0 * 1024.0));
     if (V <= 0 || N <= 0) drv::fail("empty system matrix", 2);
+    if (tune) {
+        const char* home = std::getenv("HOME");
+        const std::string cache = std::string(home ? home : ".") + "/.cache/sysmat_tiles.txt";
+        SmPeOptions best;
+        double ns = 0.0;
+        const int rc = sm_pe_autotune(P.col.data(), P.det.data(), P.img.data(), &o, 64, cache.c_str(), &best, &ns);
+        if (rc < 0) {
+            std::fprintf(stderr, "autotune failed (%s); using the default tile\n", sm_last_error());
+        } else {
+            o = best;
+            std::printf("voxel tile %d x %d x %d (%.2f ns/element on the sample%s)\n", o.tile_x, o.tile_y, o.tile_z,
+                        ns, rc == 1 ? ", cached" : "");
+        }
+    }
     if (csr_rel >= 0.0) {
         const std::string dense_name = drv::shift_name("PE_SysMat", P.img);
         const std::string path = dense_name.substr(0, dense_name.size() - 7) + ".csr";  // .sysmat -> .csr

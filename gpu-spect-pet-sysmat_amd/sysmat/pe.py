@@ -72,6 +72,19 @@ def pe_compute(geo: Geometry, **kw) -> np.ndarray:
     return dst
 
 
+def autotune(geo: Geometry, sample_rows: int = 64, cache_path: Optional[str] = None, **kw):
+    """Fastest voxel tile for this geometry and sampling (sm_pe_autotune); returns
+    (tile, ns per element, from_cache)."""
+    o = options(**kw)
+    best = SmPeOptions()
+    ns = ctypes.c_double()
+    col, det, img = _f32(geo.col), _f32(geo.det), _padded_img(geo.img)
+    rc = check(load().sm_pe_autotune(ptr(col), ptr(det), ptr(img), ctypes.byref(o), int(sample_rows),
+                                     None if cache_path is None else cache_path.encode(), ctypes.byref(best),
+                                     ctypes.byref(ns)))
+    return (best.tile_x, best.tile_y, best.tile_z), ns.value, rc == 1
+
+
 class PePlan:
     """Geometry resident in HBM; launches detector blocks into device tensors.
 

@@ -72,6 +72,14 @@ int sm_pe_compute(const float* col, const float* det, const float* img, const Sm
 int sm_pe_compute_device(const float* col, const float* det, const float* img, const SmPeOptions* o,
                          float* dst_dev, long long row_stride, void* stream);
 
+/* Tile-shape autotuner (no counterpart in the reference): times candidate voxel
+ * tiles on ~sample_rows detector rows spread over the detector (sampling and
+ * image as in `base`) and returns `base` with the fastest tile in `best`.
+ * cache_path (nullable): text cache keyed by geometry, image, sampling and GPU
+ * architecture; a cached answer returns 1 without timing, a fresh one 0. */
+int sm_pe_autotune(const float* col, const float* det, const float* img, const SmPeOptions* base, int sample_rows,
+                   const char* cache_path, SmPeOptions* best, double* best_ns_per_element);
+
 /* Plans keep the geometry tables resident in HBM for repeated launches. */
 typedef struct SmPePlan SmPePlan;
 int sm_pe_plan_create(const float* col, const float* det, const float* img, const SmPeOptions* o, SmPePlan** plan);

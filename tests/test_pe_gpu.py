@@ -139,3 +139,18 @@ def test_bad_arguments_raise(lib):
     g = params.tiny(seed=9)
     with pytest.raises(pe.SysmatError if hasattr(pe, "SysmatError") else Exception):
         pe_compute(g, det_range=(0, g.n_det + 5))
+
+
+def test_autotune_picks_a_tile_and_caches(lib, tmp_path):
+    """The tuner returns a candidate tile, a second call answers from the cache, and the
+    tuned tile gives the same matrix bit for bit (every element's work is tile-independent)."""
+    from sysmat import pe as pe_mod
+    g = params.tiny(seed=8, n_det_side=4, n_layers=2, image=(8, 8, 8), holes_side=3)
+    cache = str(tmp_path / "tiles.txt")
+    tile, ns, cached = pe_mod.autotune(g, sample_rows=8, cache_path=cache, sub=(2, 2, 2))
+    assert not cached and ns > 0 and all(1 <= t <= 16 for t in tile)
+    tile2, ns2, cached2 = pe_mod.autotune(g, sample_rows=8, cache_path=cache, sub=(2, 2, 2))
+    assert cached2 and tile2 == tile and abs(ns2 - ns) < 1e-3
+    a = pe_mod.pe_compute(g, sub=(2, 2, 2), tile=tile)
+    b = pe_mod.pe_compute(g, sub=(2, 2, 2), tile=(4, 4, 4))
+    assert np.array_equal(a, b)
