@@ -602,6 +602,7 @@ struct SmSurrogate {
     DeviceBuf<ScPair> keep;    // [N][n_keep] ranked sites
     DeviceBuf<int> keep_cnt;   // [N]
     DeviceBuf<SgRow> srow;     // [N]
+    DeviceBuf<float> ftab;     // [N][kSgTab + 1] angular factor of far sites
     SgParams q{};
     int n_near = 0, n_ring = 0;
 };
@@ -689,7 +690,30 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     p.mode = 0;
     p.det_begin = 0;
     p.det_count = n;
-    SurrogateRank k{dcand.ptr, std::max(1, n_cand), n_keep};
+    // angular factor per detector: KN shape x window probability at the scattered energy
+    {
+        std::vector<float> tab((size_t)n * (kSgTab + 1));
+        const double e0 = S->G.base.e0, ea = e0 / 511.0;
+        for (int r = 0; r < n; ++r) {
+            const ScRow& R = S->G.hrows[r];
+            for (int i = 0; i <= kSgTab; ++i) {
+                const double c = -1.0 + 2.0 * i / kSgTab;
+                const double a = ea * (1.0 - c), c2 = 1.0 + c * c;
+                const double kn = c2 / ((1.0 + a) * (1.0 + a)) * (1.0 + a * a / (c2 * (1.0 + a)));
+                const double es = e0 / (1.0 + a);
+                const double res_s = R.res * std::sqrt(es / e0);
+                const double k2 = 2.0 * res_s / 2.35482;
+                double pw = 0.0;
+                if (!(es * (1.0 + k2) <= R.lo || es * (1.0 - k2) >= R.hi)) {
+                    const double s2 = res_s / 2.35482 * es * std::sqrt(2.0);
+                    pw = 0.5 * (std::erf((R.hi - es) / s2) - std::erf((R.lo - es) / s2));
+                }
+                tab[(size_t)r * (kSgTab + 1) + i] = (float)(kn * pw);
+            }
+        }
+        if ((rc = S->ftab.upload(tab)) < 0) return fail(rc);
+    }
+    SurrogateRank k{dcand.ptr, std::max(1, n_cand), n_keep, S->ftab.ptr};
     if (n_cand >= 1) {
         HIP_TRY(launch_surrogate_rank(p, k, S->G.rows.ptr, S->G.cry.ptr, S->bits.ptr, S->keep.ptr, S->keep_cnt.ptr,
                                       nullptr));
@@ -751,6 +775,7 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     q.sig_y = 1.0f;
     q.phi = 0.0f;
     q.n_tiles = (b.V + 63) / 64;
+    q.ftab = S->ftab.ptr;
     *out = S;
     return SM_OK;
 }

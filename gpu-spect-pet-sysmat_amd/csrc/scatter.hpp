@@ -44,7 +44,8 @@ struct alignas(16) ScPair {
     float l_abs;              // absorption chord through r along S -> Q extended 1000 mm
     float c_fwd;              // cos(asin(R/L)) + margin: "through" test needed when |cos theta| >= this
     int site;                 // crystal index (crystal mode) or hole index (collimator mode)
-    float hy1, hy2;           // hole faces (collimator mode, as stored)
+    float hy1, hy2;           // hole faces (collimator mode, as stored); surrogate tables (crystal
+                              // mode): hy1 = weight / angular factor at the image centre
     float pad;                // surrogate tables: single-site weight at the image centre
 };
 
@@ -82,7 +83,19 @@ struct SurrogateRank {
     const int* cand;          // [n_det][n_cand] candidate sites (nearest crystals), -1 = none
     int n_cand;               // <= 256
     int n_keep;               // n_near + n_ring
+    const float* ftab;        // [n_det][kSgTab + 1] angular factor F_r(cos theta), see sg_angular
 };
+
+// Angular factor of a far site as seen by detector r: KN shape x energy-window
+// probability at the scattered energy, tabulated on cos(theta) in [-1, 1].
+constexpr int kSgTab = 64;
+__host__ __device__ inline float sg_angular(const float* __restrict__ tab, float c) {
+    float x = (c + 1.0f) * (0.5f * kSgTab);
+    x = fminf(fmaxf(x, 0.0f), (float)kSgTab);
+    const int i = min((int)x, kSgTab - 1);
+    const float f = x - (float)i;
+    return tab[i] + f * (tab[i + 1] - tab[i]);
+}
 
 hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Crystal* cry, const Hole* hol,
                                const unsigned* bits, ScPair* pairs, int* counts, hipStream_t stream);

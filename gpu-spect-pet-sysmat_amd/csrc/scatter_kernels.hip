@@ -492,6 +492,11 @@ __global__ void __launch_bounds__(256) surrogate_rank_kernel(ScParams p, Surroga
             const double g = site_term(p, R, P, q, kn_coef(p.e0), (double)(float)(R.mu_c / R.mu_pe));
             key = g > 0.0 ? (float)g : 0.0f;
             P.pad = key;
+            // weight without its angular factor at the image centre (voxel-dependent ring prior)
+            const double ax = P.sx - q.xI, ay = P.sy - q.yI, az = P.sz - q.zI;
+            const float c0 = (float)((ax * P.bx + ay * P.by + az * P.bz) / sqrt(ax * ax + ay * ay + az * az));
+            const float f0 = k.ftab ? sg_angular(k.ftab + (long long)r * (kSgTab + 1), c0) : 0.0f;
+            P.hy1 = f0 > 1e-30f ? key / f0 : 0.0f;
         }
     }
     s_key[t] = key;

@@ -27,8 +27,17 @@ __device__ inline void sg_features(const SgParams& q, const SgRow& R, const ScPa
     f[3] = R.f3;
     f[4] = R.f4;
     f[5] = R.f5;
+    // ring prior: sites n_near.. with their weight re-evaluated at this voxel's scatter angle
     float ring = 0.0f;
-    for (int j = q.n_near; j < cnt; ++j) ring += pe[(long long)keep[j].site * q.V + v] * keep[j].pad;
+    const float* tab = q.ftab + (long long)r * (kSgTab + 1);
+    for (int j = q.n_near; j < cnt; ++j) {
+        const ScPair& S = keep[j];
+        const float wg = S.hy1;
+        if (wg == 0.0f) continue;
+        const float ax = (float)S.sx - xI, ay = (float)S.sy - yI, az = (float)S.sz - zI;
+        const float c = (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
+        ring += pe[(long long)S.site * q.V + v] * wg * sg_angular(tab, c);
+    }
     const float inv_s = 1.0f / q.s_scale;
     f[6] = 0.1f * __logf(s_near * inv_s + 1e-6f);
     f[7] = 0.1f * __logf(ring * inv_s + 1e-6f);

@@ -37,6 +37,7 @@ struct SgParams {
     float tol, floor_abs;
     int n_members;
     long long n_tiles;          // 64-voxel tiles per row
+    const float* ftab;          // [n_det][kSgTab + 1] angular factor tables
 };
 
 hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
