@@ -150,6 +150,9 @@ long long collimator_words(int nh, int n) { return ((long long)n * n * nh + 31) 
 
 // Run prep + accumulation for detector rows [det_begin, det_begin + count) in
 // blocks that keep the pair table under `pair_budget` bytes.
+static int run_blocks(const ScatterGeom& G, ScParams p, const unsigned* bits_dev, const float* pe_dev, int det_begin,
+                      int det_count, float* dst_dev, long long row_stride, hipStream_t s);
+
 static int run_scatter(ScatterGeom& G, int mode, const unsigned* bits_dev, const float* pe_dev, int det_begin,
                        int det_count, float* dst_dev, long long row_stride, hipStream_t s) {
     ScParams p = G.base;
@@ -160,6 +163,72 @@ static int run_scatter(ScatterGeom& G, int mode, const unsigned* bits_dev, const
             HIP_TRY(hipMemsetAsync(dst_dev + (long long)r * row_stride, 0, sizeof(float) * p.V, s));
         return SM_OK;
     }
+    return run_blocks(G, p, bits_dev, pe_dev, det_begin, det_count, dst_dev, row_stride, s);
+}
+
+// Collimator plate cells resident on the device for one (cell size, keep distance).
+struct CellModel {
+    double h = -1.0, keep = 0.0;
+    std::vector<Cell> cells;
+    DeviceBuf<Cell> dcells;
+    DeviceBuf<CellPlate> dplates;
+    DeviceBuf<CellHole> dholes;
+    int n_plates = 0;
+};
+
+static int prepare_cells(const float* col, float fov2c, double h, double keep, CellModel& M) {
+    if (M.h == h && M.keep == keep && M.dcells.ptr) return SM_OK;
+    std::vector<CellPlate> plates;
+    std::vector<CellHole> holes;
+    const std::string err = build_cells(col, fov2c, h, keep, M.cells, plates, holes);
+    if (!err.empty()) return set_error(SM_E_ARG, err);
+    int rc;
+    if ((rc = M.dcells.upload(M.cells)) < 0) return rc;
+    if ((rc = M.dplates.upload(plates)) < 0) return rc;
+    if ((rc = M.dholes.upload(holes)) < 0) return rc;
+    M.n_plates = (int)plates.size();
+    M.h = h;
+    M.keep = keep;
+    return SM_OK;
+}
+
+// Cell-model collimator scatter: cells in chunks whose weight table stays under
+// 4 GiB (or `per_chunk` cells); the first chunk writes, later chunks accumulate.
+static int run_scatter_cells(ScatterGeom& G, CellModel& M, int det_begin, int det_count, float* dst_dev,
+                             long long row_stride, int per_chunk, hipStream_t s) {
+    ScParams p = G.base;
+    p.mode = 3;
+    p.cplates = M.dplates.ptr;
+    p.choles = M.dholes.ptr;
+    p.n_cplates = M.n_plates;
+    const int n_cells = (int)M.cells.size();
+    if (n_cells == 0) {
+        for (int r = 0; r < det_count; ++r)
+            HIP_TRY(hipMemsetAsync(dst_dev + (long long)r * row_stride, 0, sizeof(float) * p.V, s));
+        return SM_OK;
+    }
+    const long long table_budget = 4096ll << 20;
+    int chunk = per_chunk > 0 ? per_chunk
+                              : (int)std::max(1ll, std::min<long long>(n_cells, table_budget / (p.V * (long long)sizeof(float))));
+    chunk = std::min(chunk, n_cells);
+    DeviceBuf<float> W;
+    int rc;
+    if ((rc = W.alloc((size_t)chunk * p.V)) < 0) return rc;
+    for (int c0 = 0; c0 < n_cells; c0 += chunk) {
+        const int cnt = std::min(chunk, n_cells - c0);
+        p.cells = M.dcells.ptr + c0;
+        p.max_sites = cnt;
+        p.accumulate = c0 > 0 ? 1 : 0;
+        HIP_TRY(launch_cell_weights(p, cnt, W.ptr, s));
+        if ((rc = run_blocks(G, p, nullptr, W.ptr, det_begin, det_count, dst_dev, row_stride, s)) < 0) return rc;
+    }
+    return SM_OK;
+}
+
+// Prep + accumulation of rows [det_begin, det_begin + det_count) for the mode and
+// sites in `p`, in row blocks whose pair table stays under 2 GiB.
+static int run_blocks(const ScatterGeom& G, ScParams p, const unsigned* bits_dev, const float* pe_dev, int det_begin,
+                      int det_count, float* dst_dev, long long row_stride, hipStream_t s) {
     const long long pair_budget = 2048ll << 20;  // 2 GiB of pair records per block
     int block = (int)std::max(4ll, std::min<long long>(det_count, pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
     block = (block + 3) / 4 * 4;
@@ -295,6 +364,9 @@ struct SmScatterPlan {
     DeviceBuf<unsigned> bits_crystal, bits_collimator;  // empty: criterion evaluated inline
     int variant = 0;
     bool has_collimator = false;
+    std::vector<float> col;   // collimator record (cell model)
+    float fov2c = 0.0f;
+    CellModel cells;
 };
 
 extern "C" {
@@ -366,6 +438,11 @@ int sm_scatter_plan_create_ex(const float* col, const float* det, const float* i
     }
     if (rc >= 0 && col) {
         P->has_collimator = true;
+        const int n_layers = std::max(0, std::min(9, (int)std::floor(col[0] + 0.000001f)));
+        int n_h = 0;
+        for (int m = 0; m < n_layers; ++m) n_h += std::max(0, (int)std::floor(col[(m + 1) * 10]));
+        P->col.assign(col, col + 100 + 9 * (size_t)n_h);
+        P->fov2c = img[11];
         const long long nw = collimator_words(P->G.n_holes0, P->G.T.n_det);
         if (flags & SM_PLAN_NO_COLLIMATOR_BITMAP) {
             // criterion evaluated inline by the pair preparation
@@ -435,6 +512,46 @@ int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count
                        dst_dev, row_stride > 0 ? row_stride : plan->G.base.V, (hipStream_t)stream);
 }
 
+int sm_scatter_plan_cells(SmScatterPlan* plan, float cell_size, float keep_dist, int det_begin, int det_count,
+                          float* dst_dev, long long row_stride, int cells_per_chunk, void* stream) {
+    clear_error();
+    if (!plan || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    if (!plan->has_collimator) return set_error(SM_E_ARG, "plan created without a collimator");
+    if (!(cell_size > 0.0f)) return set_error(SM_E_ARG, "cell size must be positive");
+    const int n = plan->G.T.n_det;
+    if (det_begin < 0 || det_count <= 0 || det_begin + det_count > n)
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    int rc = use_device(plan->G.device);
+    if (rc < 0) return rc;
+    if ((rc = prepare_cells(plan->col.data(), plan->fov2c, cell_size, keep_dist, plan->cells)) < 0) return rc;
+    rc = run_scatter_cells(plan->G, plan->cells, det_begin, det_count, dst_dev,
+                           row_stride > 0 ? row_stride : plan->G.base.V, cells_per_chunk, (hipStream_t)stream);
+    if (rc < 0) return rc;
+    HIP_TRY(hipStreamSynchronize((hipStream_t)stream));
+    return SM_OK;
+}
+
+long long sm_scatter_cell_table(SmScatterPlan* plan, float cell_size, float keep_dist, double* out, long long cap) {
+    clear_error();
+    if (!plan) return set_error(SM_E_ARG, "null argument");
+    if (!plan->has_collimator) return set_error(SM_E_ARG, "plan created without a collimator");
+    if (!(cell_size > 0.0f)) return set_error(SM_E_ARG, "cell size must be positive");
+    int rc = use_device(plan->G.device);
+    if (rc < 0) return rc;
+    if ((rc = prepare_cells(plan->col.data(), plan->fov2c, cell_size, keep_dist, plan->cells)) < 0) return rc;
+    const long long nc = (long long)plan->cells.cells.size();
+    if (out) {
+        for (long long i = 0; i < std::min(nc, cap); ++i) {
+            const Cell& c = plan->cells.cells[(size_t)i];
+            out[4 * i + 0] = c.x;
+            out[4 * i + 1] = c.y;
+            out[4 * i + 2] = c.z;
+            out[4 * i + 3] = c.w;
+        }
+    }
+    return nc;
+}
+
 int sm_scatter_plan_destroy(SmScatterPlan* plan) {
     if (plan) {
         use_device(plan->G.device);
@@ -479,6 +596,30 @@ int sm_scatter_collimator(const float* col, const float* det, const float* img, 
     DeviceBuf<float> dout;
     if ((rc = dout.alloc((size_t)det_count * V)) >= 0) {
         rc = sm_scatter_plan_collimator(plan, det_begin, det_count, dout.ptr, V, nullptr);
+        if (rc >= 0) {
+            hipError_t e = hipMemcpy(dst, dout.ptr, sizeof(float) * det_count * V, hipMemcpyDeviceToHost);
+            if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "result download failed");
+        }
+    }
+    sm_scatter_plan_destroy(plan);
+    return rc;
+}
+
+// Host-memory convenience: plate-cell collimator scatter rows.
+int sm_scatter_cells(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                     float cell_size, float keep_dist, int det_begin, int det_count, float* dst, int device) {
+    clear_error();
+    if (!col || !dst) return set_error(SM_E_ARG, "null argument");
+    SmScatterPlan* plan = nullptr;
+    int rc = sm_scatter_plan_create_ex(col, det, img, phy, rotation, 0,
+                                       SM_PLAN_NO_CRYSTAL_BITMAP | SM_PLAN_NO_COLLIMATOR_BITMAP, nullptr, nullptr,
+                                       device, &plan);
+    if (rc < 0) return rc;
+    const long long V = plan->G.base.V, N = plan->G.T.n_det;
+    if (det_count <= 0) det_count = (int)N - det_begin;
+    DeviceBuf<float> dout;
+    if ((rc = dout.alloc((size_t)det_count * V)) >= 0) {
+        rc = sm_scatter_plan_cells(plan, cell_size, keep_dist, det_begin, det_count, dout.ptr, V, 0, nullptr);
         if (rc >= 0) {
             hipError_t e = hipMemcpy(dst, dout.ptr, sizeof(float) * det_count * V, hipMemcpyDeviceToHost);
             if (e != hipSuccess) rc = set_error(SM_E_DEVICE, "result download failed");

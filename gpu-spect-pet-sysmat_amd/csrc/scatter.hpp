@@ -15,6 +15,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "cells.hpp"
 #include "tables.hpp"
 
 namespace sysmat {
@@ -33,7 +34,7 @@ struct alignas(16) ScRow {
     float pad0, pad1, pad2;
 };
 
-// Per (r, site) pair, 192 B.
+// Per (r, site) pair, 192 B.  A_pe / A_c also hold the plate path of a cell site.
 struct alignas(16) ScPair {
     double sx, sy, sz;        // site (world)
     double ssx, ssy, ssz;     // site in r's frame
@@ -61,7 +62,8 @@ struct ScParams {
     int max_sites;            // pair table stride per row
     long long row_stride;     // floats between detector rows of dst
     long long V;              // voxels (PE matrix row length)
-    int mode;                 // 0 crystal scatter, 1 collimator (combined), 2 collimator (ScatterGen_Collimator)
+    int mode;                 // 0 crystal scatter, 1 collimator (combined), 2 collimator (ScatterGen_Collimator),
+                              // 3 collimator plate cells (source weights in the `pe` table, [cell][voxel])
     double w_col;             // collimator source factor Col[17] / Col[15]
     double fov2c;
     int n_tiles;              // voxel tiles of 64
@@ -74,6 +76,11 @@ struct ScParams {
     const int2* items;        // list mode: wave w of block b handles items[4b + w] = (rr, voxel tile)
     long long n_items;
     int accumulate;           // 1: dst += element (e.g. crystal scatter onto the collimator term)
+    // collimator cells (mode 3)
+    const Cell* cells;        // this launch's cells; pair.site indexes them and the weight table rows
+    const CellPlate* cplates;
+    const CellHole* choles;
+    int n_cplates;
 };
 
 // Scatter-surrogate near field: per detector the n_near + n_ring sites with the
@@ -103,6 +110,7 @@ hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pa
                           const float* pe, float* dst, hipStream_t stream);
 hipError_t launch_surrogate_rank(const ScParams& p, const SurrogateRank& k, const ScRow* rows, const Crystal* cry,
                                  const unsigned* bits, ScPair* keep, int* counts, hipStream_t stream);
+hipError_t launch_cell_weights(const ScParams& p, int n_cells, float* weights, hipStream_t stream);
 hipError_t launch_bitmap_crystal(const Crystal* cry, int n, unsigned* words, hipStream_t stream);
 hipError_t launch_bitmap_collimator(const Hole* hol, int n_holes, const Crystal* cry, int n, unsigned* words,
                                     hipStream_t stream);

@@ -5,7 +5,9 @@
 //   scatter_prep_kernel       per (detector r, site) constants; the reference
 //                             recomputes them for every voxel (scatter.cu:1232-1417)
 //   scatter_kernel            crystalScatterSysMatCuda / collimatorScatterSysMatCuda
-//                             (scatter.cu:1077-1437, 1544-1936)
+//                             (scatter.cu:1077-1437, 1544-1936), and the plate-cell
+//                             collimator model (sites = cells, weights from
+//                             cell_weight_kernel)
 //
 // The bitmap kernels reproduce the reference bits exactly: IEEE FP32 division
 // and square root, no FMA contraction, the reference's operation order.  The
@@ -195,6 +197,10 @@ __device__ inline bool make_pair(const ScParams& p, const ScRow& R, int r, int s
         const Crystal K = cry[s];
         sx = K.cx; sy = ycen(K, p.fov2c); sz = K.cz;
         alive = K.mu_t > 0.01f;  // a crystal with mu_t <= 0.01 has an all-zero PE row
+    } else if (p.mode == 3) {
+        const Cell C = p.cells[s];
+        sx = C.x; sy = C.y; sz = C.z;
+        alive = true;
     } else {
         const Hole H = hol[s];
         sx = H.x;
@@ -234,6 +240,30 @@ __device__ inline bool make_pair(const ScParams& p, const ScRow& R, int r, int s
     P.c_fwd = (q >= 1.0) ? -2.0f : (float)(sqrt(1.0 - q * q) - 1e-6);
     // attenuators (scatter.cu:1343-1401 / 1839-1897)
     double ape = 0.0, ac = 0.0;
+    if (p.mode == 3) {
+        // plate path to the detector centre, then the crystals passing the
+        // bounding-sphere criterion seen from the cell (FP64)
+        const PathAtt o = plate_path(p.cplates, p.n_cplates, p.choles, sx, sy, sz, R.cx, R.cy, R.cz);
+        ape = o.pe;
+        ac = o.c;
+        const double ujx = bx / L, ujy = by / L, ujz = bz / L, cj = R.rdet / L;
+        for (int a = 0; a < n; ++a) {
+            if (a == r) continue;
+            const Crystal A = cry[a];
+            const double kx = (double)A.cx - sx, ky = ycen(A, p.fov2c) - sy, kz = (double)A.cz - sz;
+            const double lk = sqrt(kx * kx + ky * ky + kz * kz);
+            const double hx = A.hx, hy = A.hy, hz = A.hz;
+            const double ck = sqrt(hx * hx + hy * hy + hz * hz) / lk;
+            const double ex = kx / lk - ujx, ey = ky / lk - ujy, ez = kz / lk - ujz;
+            if (!(sqrt(ex * ex + ey * ey + ez * ez) <= ck + cj)) continue;
+            const double c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
+            ape += c * (double)A.mu_pe;
+            ac += c * (double)A.mu_c;
+        }
+        P.A_pe = (float)ape;
+        P.A_c = (float)ac;
+        return true;
+    }
     const int i0 = s < r ? s : r, j0 = s < r ? r : s;
     CrossIJ cij{};
     CrossHJ chj{};
@@ -348,6 +378,7 @@ __device__ inline VoxelPt voxel_of(const ScParams& p, const ScRow& R, long long 
 __device__ inline double site_weight(const ScParams& p, const ScRow& R, const ScPair& S, const VoxelPt& q,
                                      float pe_v) {
     if (p.mode == 0) return (double)(pe_v * (float)(R.mu_c / R.mu_pe));
+    if (p.mode == 3) return (double)pe_v;
     const double lx = q.xI - S.sx, ly = q.yI - S.sy, lz = q.zI - S.sz;
     const double Li = sqrt(lx * lx + ly * ly + lz * lz);
     const double dyh = (double)S.hy2 - (double)S.hy1;
@@ -456,7 +487,8 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
         const ScPair* P = pairs + (long long)rr * p.max_sites;
         for (int j = 0; j < n_sites; ++j) {
             const ScPair S = P[j];
-            const double w = site_weight(p, R, S, q, p.mode == 0 ? pe[(long long)S.site * p.V + v] : 0.0f);
+            const bool table = p.mode == 0 || p.mode == 3;
+            const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
             acc += site_term(p, R, S, q, kc, w);
         }
     }
@@ -465,6 +497,27 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
         if (p.accumulate) *o += (float)acc;
         else *o = (float)acc;
     }
+}
+
+// Source weights of the collimator cells: W[c][v] = w_c / |c - I|^2 exp(-A_t(I -> c)),
+// I the voxel centre (world).  Block b covers cell b / tiles, voxels 256 (b % tiles) + lane.
+__global__ void __launch_bounds__(256) cell_weight_kernel(ScParams p, long long tiles, float* __restrict__ W) {
+    const long long b = blockIdx.x;
+    const int c = (int)(b / tiles);
+    const long long v = (b % tiles) * 256 + threadIdx.x;
+    if (v >= p.V) return;
+    const int X = p.X, Y = p.Y;
+    const int iz = (int)(v / ((long long)X * Y));
+    const int rem = (int)(v % ((long long)X * Y));
+    const int iy = rem / X, ix = rem % X;
+    const double x0 = ((double)ix - (double)X / 2.0 + 0.5) * p.wx + p.sx;
+    const double y0 = ((double)iy - (double)Y / 2.0 + 0.5) * p.wy + p.sy;
+    const double zI = ((double)iz - (double)p.Z / 2.0 + 0.5) * p.wz + p.sz;
+    const double xI = x0 * p.rc - y0 * p.rs, yI = x0 * p.rs + y0 * p.rc;
+    const Cell C = p.cells[c];
+    const double dx = C.x - xI, dy = C.y - yI, dz = C.z - zI;
+    const PathAtt a = plate_path(p.cplates, p.n_cplates, p.choles, xI, yI, zI, C.x, C.y, C.z);
+    W[(long long)c * p.V + v] = (float)(C.w / (dx * dx + dy * dy + dz * dz) * exp(-a.t));
 }
 
 // Scatter-surrogate ranking: one workgroup per detector row; every candidate
@@ -528,6 +581,23 @@ __global__ void __launch_bounds__(256) surrogate_rank_kernel(ScParams p, Surroga
         while (c < k.n_keep && c < 256 && s_key[c] >= 0.0f) ++c;
         counts[rr] = c;
     }
+}
+
+hipError_t launch_cell_weights(const ScParams& p, int n_cells, float* weights, hipStream_t stream) {
+    if (n_cells <= 0 || p.V <= 0) return hipSuccess;
+    const long long tiles = (p.V + 255) / 256;
+    const long long max_blocks = (1LL << 32) / 256 - 8;
+    const int per = (int)std::max(1LL, std::min<long long>(n_cells, max_blocks / tiles));
+    for (int c0 = 0; c0 < n_cells; c0 += per) {
+        ScParams q = p;
+        q.cells = p.cells + c0;
+        const int cnt = std::min(per, n_cells - c0);
+        hipLaunchKernelGGL(cell_weight_kernel, dim3((unsigned)(tiles * cnt)), dim3(256), 0, stream, q, tiles,
+                           weights + (long long)c0 * p.V);
+        const hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
 }
 
 hipError_t launch_bitmap_crystal(const Crystal* cry, int n, unsigned* words, hipStream_t stream) {

@@ -30,6 +30,10 @@
 
 static void usage(const char* argv0, bool with_cuda) {
     std::printf("  (extension: -pe-rotation0 feeds every rotation the rotation-0 PE block, as the reference)\n");
+#if SCATTER_MODE != 2
+    std::printf("  (extension: -cells SIZE KEEP replaces the hole-site collimator term by plate cells of SIZE mm\n"
+                "   kept within KEEP mm of a hole wall, KEEP < 0: every cell)\n");
+#endif
 #if SCATTER_MODE == 0
     std::printf("Usage: %s [-PE PE_SysMat_path] [-GeoCrystal GeometryRelationShip_Crystal2Crystal_path] "
                 "[-GeoCollimator GeometryRelationShip_Collimator2Crystal_path]%s\n", argv0, with_cuda ? " [-cuda GPU_ID]" : " ");
@@ -50,6 +54,7 @@ int main(int argc, char* argv[]) {
     std::string geo_collimator = "GeometryRelationShip_Collimator2Crystal";
     int cuda_id = 0;
     bool pe_rotation0 = false;
+    float cell_size = 0.0f, cell_keep = -1.0f;
     for (int i = 1; i < argc; ++i) {
         if (std::strcmp(argv[i], "-PE") == 0 && i + 1 < argc) {
             fname_pe = argv[++i];
@@ -67,6 +72,15 @@ int main(int argc, char* argv[]) {
             cuda_id = std::atoi(argv[++i]);
         } else if (std::strcmp(argv[i], "-pe-rotation0") == 0) {
             pe_rotation0 = true;
+#if SCATTER_MODE != 2
+        } else if (std::strcmp(argv[i], "-cells") == 0 && i + 2 < argc) {
+            cell_size = (float)std::atof(argv[++i]);
+            cell_keep = (float)std::atof(argv[++i]);
+            if (!(cell_size > 0.0f)) {
+                std::fprintf(stderr, "-cells: cell size must be positive\n");
+                return EXIT_FAILURE;
+            }
+#endif
         } else if (std::strcmp(argv[i], "-h") == 0 || std::strcmp(argv[i], "--help") == 0) {
             usage(argv[0], true);
             return 0;
@@ -105,11 +119,31 @@ int main(int argc, char* argv[]) {
         float* dst = out.data() + (size_t)rot * per_rot;
         const float* pe_rot = pe.data() + (pe_rotation0 ? 0 : (size_t)rot * per_rot);
 #if SCATTER_MODE == 0
-        int q = sm_scatter(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), pe_rot, geo_crystal.c_str(),
+        int q;
+        if (cell_size > 0.0f) {  // crystal term (bitmap file semantics) + plate-cell collimator term
+            q = sm_scatter_crystal_only(P.det.data(), P.img.data(), P.phy.data(), pe_rot, geo_crystal.c_str(), dst,
+                                        cuda_id);
+            if (q >= 0) {
+                std::vector<float> cells(per_rot);
+                const int c = sm_scatter_cells(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), rot, cell_size,
+                                               cell_keep, 0, N, cells.data(), cuda_id);
+                if (c < 0) q = c;
+                for (size_t i = 0; q >= 0 && i < per_rot; ++i) dst[i] = cells[i] + dst[i];
+            }
+        } else {
+            q = sm_scatter(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), pe_rot, geo_crystal.c_str(),
                            geo_collimator.c_str(), dst, cuda_id);
+        }
 #elif SCATTER_MODE == 1
-        int q = sm_scatter_collimator_only(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), pe_rot,
+        int q;
+        if (cell_size > 0.0f) {
+            q = sm_scatter_cells(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), rot, cell_size, cell_keep, 0,
+                                 N, dst, cuda_id);
+            if (q >= 0) q = (int)V;
+        } else {
+            q = sm_scatter_collimator_only(P.col.data(), P.det.data(), P.img.data(), P.phy.data(), pe_rot,
                                            geo_collimator.c_str(), dst, cuda_id);
+        }
 #else
         int q = sm_scatter_crystal_only(P.det.data(), P.img.data(), P.phy.data(), pe_rot, geo_crystal.c_str(), dst,
                                         cuda_id);

@@ -75,6 +75,11 @@ SIGNATURES = {
     "sm_scatter_plan_create_ex": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int, _VP, _VP,
                                                  ctypes.c_int, ctypes.POINTER(_VP)]),
     "sm_scatter_plan_crystal_rows": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, ctypes.c_longlong, _VP]),
+    "sm_scatter_plan_cells": (ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _VP,
+                                             ctypes.c_longlong, ctypes.c_int, _VP]),
+    "sm_scatter_cell_table": (ctypes.c_longlong, [_VP, ctypes.c_float, ctypes.c_float, _VP, ctypes.c_longlong]),
+    "sm_scatter_cells": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                        ctypes.c_int, _VP, ctypes.c_int]),
     "sm_surrogate_create": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                            ctypes.c_int, ctypes.POINTER(_VP)]),
     "sm_surrogate_set_norm": (ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_float, ctypes.c_float,

@@ -74,6 +74,16 @@ def scatter_collimator(geo: Geometry, bits: Optional[np.ndarray] = None, rotatio
     return dst
 
 
+def scatter_cells(geo: Geometry, cell_size: float = 0.5, keep_dist: float = 1.0, rotation: int = 0,
+                  det_range: Optional[Tuple[int, int]] = None, device: int = 0) -> np.ndarray:
+    """Plate-cell collimator scatter rows (host memory); see ScatterPlan.collimator_cells."""
+    b, e = det_range if det_range is not None else (0, geo.n_det)
+    dst = np.zeros((e - b, geo.n_vox), np.float32)
+    check(load().sm_scatter_cells(ptr(_f32(geo.col)), ptr(_f32(geo.det)), ptr(_img(geo.img)), ptr(_f32(geo.phy)),
+                                  rotation, cell_size, keep_dist, b, e - b, ptr(dst), device))
+    return dst
+
+
 def scatter(parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics, PE_SysMat,
             FnameGeoCrystal: str, FnameGeoCollimator: str, dst: np.ndarray, cuda_id: int = 0) -> int:
     """Drop-in for the combined reference scatter(); ``dst`` (N*V float32) is written in place."""
@@ -128,6 +138,22 @@ class ScatterPlan:
         s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
         check(self._lib.sm_scatter_plan_collimator(self._h, det_begin, det_count, ptr(dst_dev), row_stride,
                                                    ctypes.c_void_p(s) if s else None))
+
+    def collimator_cells(self, det_begin: int, det_count: int, dst_dev, cell_size: float = 0.5,
+                         keep_dist: float = 1.0, row_stride: int = 0, cells_per_chunk: int = 0, stream=None) -> None:
+        """Collimator scatter with the plates cut into ``cell_size`` mm cells (each a Compton
+        site; in-leg and out-leg plate attenuation included).  ``keep_dist`` < 0 keeps every
+        material cell, else the cells within ``keep_dist`` mm of a hole wall."""
+        s = 0 if stream is None else int(getattr(stream, "cuda_stream", stream))
+        check(self._lib.sm_scatter_plan_cells(self._h, cell_size, keep_dist, det_begin, det_count, ptr(dst_dev),
+                                              row_stride, cells_per_chunk, ctypes.c_void_p(s) if s else None))
+
+    def cell_table(self, cell_size: float = 0.5, keep_dist: float = 1.0) -> np.ndarray:
+        """Kept cells as an (n, 4) float64 array: x, y (FOV2C added), z, mu_c vol / (4 pi)."""
+        n = int(check(self._lib.sm_scatter_cell_table(self._h, cell_size, keep_dist, None, 0)))
+        out = np.zeros((max(n, 1), 4), np.float64)
+        check(self._lib.sm_scatter_cell_table(self._h, cell_size, keep_dist, ptr(out), n))
+        return out[:n]
 
     def close(self) -> None:
         if getattr(self, "_h", None):

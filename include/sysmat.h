@@ -148,6 +148,24 @@ int sm_scatter_plan_crystal(SmScatterPlan* plan, const float* pe_dev, int det_be
                             long long row_stride, void* stream);
 int sm_scatter_plan_collimator(SmScatterPlan* plan, int det_begin, int det_count, float* dst_dev,
                                long long row_stride, void* stream);
+/* Collimator scatter with the plates cut into cells of cell_size mm, each cell a
+ * Compton site.  No counterpart in the reference, whose collimator scatter puts
+ * one site at each layer-0 hole with a virtual-disc weight and no collimator
+ * attenuation (scatter.cu:1544-1936, replaced by sm_scatter_plan_collimator).
+ * Cell weight mu_c vol / (4 pi d^2) exp(-plate path in), plate path out scaled to
+ * E' like the crystal attenuators; the rest is the hole-site physics.  keep_dist < 0
+ * keeps every material cell, else cells whose centre lies within keep_dist mm of a
+ * hole wall of their plate.  cells_per_chunk 0: automatic (cell-weight table
+ * <= 4 GiB).  Synchronous on `stream`.  The plan must have a collimator. */
+int sm_scatter_plan_cells(SmScatterPlan* plan, float cell_size, float keep_dist, int det_begin, int det_count,
+                          float* dst_dev, long long row_stride, int cells_per_chunk, void* stream);
+/* The kept cells of (cell_size, keep_dist): up to `cap` records of 4 doubles
+ * (x, y with FOV2C, z, mu_c vol / (4 pi)) into `out` (may be NULL); returns the count. */
+long long sm_scatter_cell_table(SmScatterPlan* plan, float cell_size, float keep_dist, double* out, long long cap);
+/* Host-memory form of sm_scatter_plan_cells: rows [det_begin, det_begin + det_count)
+ * (det_count <= 0: to the last detector) of rotation `rotation` into dst (rows x V). */
+int sm_scatter_cells(const float* col, const float* det, const float* img, const float* phy, int rotation,
+                     float cell_size, float keep_dist, int det_begin, int det_count, float* dst, int device);
 int sm_scatter_plan_destroy(SmScatterPlan* plan);
 
 /* ---- sparse output stage (no counterpart in the reference, whose only output

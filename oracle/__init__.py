@@ -78,6 +78,11 @@ def _init_scatter(L) -> None:
     L.or_bitmap_collimator.argtypes = [_f32p, _f32p, _u32p, ll]
     L.or_scatter_crystal.argtypes = [_f32p, _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.POINTER(ScOpts), i]
     L.or_scatter_collimator.argtypes = [_f32p, _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.POINTER(ScOpts), i]
+    L.or_cells.argtypes = [_f32p, ctypes.c_float, ctypes.c_double, ctypes.c_double, vp, ll]
+    L.or_cells.restype = ll
+    L.or_plate_path_f64.argtypes = [_f32p, ctypes.c_float, _f64p, _f64p]
+    L.or_scatter_cells.argtypes = [_f32p, _f32p, _f32p, _f32p, ctypes.c_double, ctypes.c_double, _f32p,
+                                   ctypes.POINTER(ScOpts), i]
     L.or_kn_riemann_f32.argtypes = [ctypes.c_float] * 3
     L.or_kn_riemann_f32.restype = ctypes.c_float
     L.or_kn_riemann_f64.argtypes = [ctypes.c_double] * 3
@@ -139,6 +144,36 @@ def scatter_collimator(geo, bits: np.ndarray, det_range=None, precision="f32", v
     lib().or_scatter_collimator(np.ascontiguousarray(geo.col, np.float32), np.ascontiguousarray(geo.det, np.float32),
                                 _sc_img(geo, rot), np.ascontiguousarray(geo.phy, np.float32),
                                 np.ascontiguousarray(bits), dst, ctypes.byref(o), 1 if precision == "f64" else 0)
+    return dst.reshape(e - b, V)
+
+
+def cells(geo, h: float, keep: float) -> np.ndarray:
+    """Collimator plate cells (x, y with FOV2C, z, mu_c vol / 4 pi), (n, 4) float64."""
+    col = np.ascontiguousarray(geo.col, np.float32)
+    n = lib().or_cells(col, float(geo.img[11]), h, keep, None, 0)
+    out = np.zeros((max(n, 1), 4), np.float64)
+    lib().or_cells(col, float(geo.img[11]), h, keep, out.ctypes.data, n)
+    return out[:n]
+
+
+def plate_path(geo, a, b) -> np.ndarray:
+    """mu_t, mu_pe, mu_c weighted path lengths of segment a -> b through the plates and holes."""
+    ab = np.ascontiguousarray(np.concatenate([np.asarray(a, np.float64), np.asarray(b, np.float64)]))
+    out = np.zeros(3, np.float64)
+    lib().or_plate_path_f64(np.ascontiguousarray(geo.col, np.float32), float(geo.img[11]), ab, out)
+    return out
+
+
+def scatter_cells(geo, h: float, keep: float, det_range=None, precision="f64", rot=0, threads=0) -> np.ndarray:
+    """Plate-cell collimator scatter rows (the cell model of the shipped library's
+    sm_scatter_plan_cells; no reference kernel)."""
+    b, e = det_range if det_range is not None else (0, geo.n_det)
+    V = geo.n_vox
+    dst = np.zeros((e - b) * V, np.float32)
+    o = ScOpts(b, e, threads, 0)
+    lib().or_scatter_cells(np.ascontiguousarray(geo.col, np.float32), np.ascontiguousarray(geo.det, np.float32),
+                           _sc_img(geo, rot), np.ascontiguousarray(geo.phy, np.float32), h, keep, dst,
+                           ctypes.byref(o), 1 if precision == "f64" else 0)
     return dst.reshape(e - b, V)
 
 

@@ -8,6 +8,7 @@
 //   collimatorScatterSysMatCuda             scatter.cu:1544-1936 (virtual_r 0.1, projected disc)
 //       and its ScatterGen_Collimator variant (ScatterGen_Collimator/scatter.cu:1056-1436,
 //       virtual_r 0.2, solid angle (r^2 + dy r) / (4 L^2), :1212-1215)
+//   the plate-cell collimator model (no reference kernel; cells_run below)
 // One element (detector r, voxel v) sums the scatter sites (crystals k != r,
 // or layer-0 holes) in ascending order; the reference's atomicAdd order is
 // nondeterministic, so the FP32 sum order is a free choice.  The reference's
@@ -15,6 +16,7 @@
 // attenuator list: the same set in the same order, independent of v.
 #include <omp.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -147,7 +149,7 @@ static Row<R> make_row(const float* det, const float* img, const float* phy, int
 // term (prob_Compton_*).  Returns the product in the reference's mixed precision.
 template <class R>
 static R site_term(const Row<R>& D, const float* det, const float* img, R xI, R yI, R zI, R xS, R yS, R zS,
-                   R weight, const std::vector<int>& atten) {
+                   R weight, const std::vector<int>& atten, R plate_pe = R(0.0f), R plate_c = R(0.0f)) {
     using W = typename Wide<R>::type;
     const R fov2c = R(img[11]);
     // single sub-cell (divide 1x1x1): the sub-point is the crystal centre
@@ -222,6 +224,8 @@ static R site_term(const Row<R>& D, const float* det, const float* img, R xI, R 
                         R(0.5f) * tk, R(0.5f) * hk);
         att = att + c * tot;
     }
+    // collimator plate path of a cell site (zero for crystal and hole sites), mu scaled to E'
+    att = att + plate_pe * D.e0 * D.e0 * D.e0 / es / es / es + plate_c * D.e0 / es;
     // inside the detecting crystal, before the (single) sub-cell
     const R a1 = chord_box_open(xSs, ySs, zSs, xDs, yDs, zDs, -D.w / R(2.0f), -D.t / R(2.0f), -D.h / R(2.0f),
                                 D.w / R(2.0f), D.t / R(2.0f), D.h / R(2.0f));
@@ -357,6 +361,213 @@ static void collimator_run(const float* col, const float* det, const float* img,
     }
 }
 
+// ---- collimator plate cells (a model of this build, no reference kernel) ---
+// Plates, holes and cells from the collimator record; cells in (y, z, x) order.
+// A cell of plate m is a site when its centre is outside every hole of m and,
+// for keep >= 0, within keep mm (x-z) of one of m's hole walls.
+struct OPlate {
+    double x1, x2, y1, y2, z1, z2, mt, mpe, mc;
+    int h0, h1;
+};
+struct OHole {
+    double x, z, r, ya, yb, mt, mpe, mc;
+};
+struct OCell {
+    double x, y, z, w;
+};
+
+static void read_collimator(const float* col, double fov2c, std::vector<OPlate>& P, std::vector<OHole>& H) {
+    const int layers = (int)std::floor(col[0] + 0.000001f);
+    int k = 0;
+    for (int m = 0; m < layers; ++m) {
+        const float* L = col + 10 * (m + 1);
+        OPlate p;
+        p.x1 = -0.5 * (double)L[1];
+        p.x2 = 0.5 * (double)L[1];
+        p.y1 = fov2c + (double)L[4] - 0.5 * (double)L[2];
+        p.y2 = fov2c + (double)L[4] + 0.5 * (double)L[2];
+        p.z1 = -0.5 * (double)L[3];
+        p.z2 = 0.5 * (double)L[3];
+        p.mt = L[5];
+        p.mpe = L[6];
+        p.mc = L[7];
+        p.h0 = k;
+        p.h1 = k + (int)std::floor(L[0]);
+        for (; k < p.h1; ++k) {
+            const float* q = col + 100 + 9 * k;
+            OHole o;
+            o.x = q[0];
+            o.z = q[3];
+            o.r = q[4];
+            o.ya = std::fmin((double)q[1] + fov2c, (double)q[2] + fov2c);
+            o.yb = std::fmax((double)q[1] + fov2c, (double)q[2] + fov2c);
+            o.mt = q[5];
+            o.mpe = q[6];
+            o.mc = q[7];
+            H.push_back(o);
+        }
+        P.push_back(p);
+    }
+}
+
+static std::vector<OCell> make_cells(const std::vector<OPlate>& P, const std::vector<OHole>& H, double h,
+                                     double keep) {
+    std::vector<OCell> out;
+    for (const OPlate& p : P) {
+        if (!(p.mc > 0.0)) continue;
+        const double sx = p.x2 - p.x1, sy = p.y2 - p.y1, sz = p.z2 - p.z1;
+        const long nx = std::max(1L, (long)std::llround(sx / h)), ny = std::max(1L, (long)std::llround(sy / h));
+        const long nz = std::max(1L, (long)std::llround(sz / h));
+        const double cx = sx / (double)nx, cy = sy / (double)ny, cz = sz / (double)nz;
+        const double w = p.mc * cx * cy * cz / (4.0 * M_PI);
+        for (long j = 0; j < ny; ++j)
+            for (long k = 0; k < nz; ++k)
+                for (long i = 0; i < nx; ++i) {
+                    const OCell c{p.x1 + (i + 0.5) * cx, p.y1 + (j + 0.5) * cy, p.z1 + (k + 0.5) * cz, w};
+                    bool air = false, near = keep < 0.0;
+                    for (int q = p.h0; q < p.h1; ++q) {
+                        const double rho = std::sqrt((c.x - H[q].x) * (c.x - H[q].x) + (c.z - H[q].z) * (c.z - H[q].z));
+                        if (rho < H[q].r && c.y >= H[q].ya && c.y <= H[q].yb) air = true;
+                        if (rho - H[q].r <= keep) near = true;
+                    }
+                    if (!air && near) out.push_back(c);
+                }
+    }
+    return out;
+}
+
+// mu-weighted path lengths (total, photoelectric, Compton) of the segment A -> B
+// through the plates: every crossing of a plate face, hole wall or hole cap cuts
+// the segment; each piece takes the material of its midpoint.
+static void plate_path(const std::vector<OPlate>& P, const std::vector<OHole>& H, const double* a, const double* b,
+                       double* out) {
+    const double d[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+    const double len = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+    std::vector<double> t{0.0, 1.0};
+    auto plane = [&](int ax, double v) {
+        if (d[ax] != 0.0) {
+            const double u = (v - a[ax]) / d[ax];
+            if (u > 0.0 && u < 1.0) t.push_back(u);
+        }
+    };
+    for (const OPlate& p : P) {
+        plane(0, p.x1); plane(0, p.x2); plane(1, p.y1); plane(1, p.y2); plane(2, p.z1); plane(2, p.z2);
+    }
+    for (const OHole& o : H) {
+        plane(1, o.ya);
+        plane(1, o.yb);
+        const double qa = d[0] * d[0] + d[2] * d[2];
+        if (qa > 0.0) {
+            const double ex = a[0] - o.x, ez = a[2] - o.z;
+            const double qb = 2.0 * (ex * d[0] + ez * d[2]), qc = ex * ex + ez * ez - o.r * o.r;
+            const double disc = qb * qb - 4.0 * qa * qc;
+            if (disc > 0.0) {
+                for (double sgn : {-1.0, 1.0}) {
+                    const double u = (-qb + sgn * std::sqrt(disc)) / (2.0 * qa);
+                    if (u > 0.0 && u < 1.0) t.push_back(u);
+                }
+            }
+        }
+    }
+    std::sort(t.begin(), t.end());
+    out[0] = out[1] = out[2] = 0.0;
+    for (size_t i = 0; i + 1 < t.size(); ++i) {
+        const double dl = (t[i + 1] - t[i]) * len;
+        if (!(dl > 0.0)) continue;
+        const double u = 0.5 * (t[i] + t[i + 1]);
+        const double x = a[0] + u * d[0], y = a[1] + u * d[1], z = a[2] + u * d[2];
+        for (const OPlate& p : P) {
+            if (x < p.x1 || x > p.x2 || y < p.y1 || y > p.y2 || z < p.z1 || z > p.z2) continue;
+            double mt = p.mt, mpe = p.mpe, mc = p.mc;
+            for (int q = p.h0; q < p.h1; ++q) {
+                const OHole& o = H[q];
+                if ((x - o.x) * (x - o.x) + (z - o.z) * (z - o.z) < o.r * o.r && y >= o.ya && y <= o.yb) {
+                    mt = o.mt;
+                    mpe = o.mpe;
+                    mc = o.mc;
+                    break;
+                }
+            }
+            out[0] += mt * dl;
+            out[1] += mpe * dl;
+            out[2] += mc * dl;
+        }
+    }
+}
+
+// Cell-model collimator scatter of rows [det_begin, det_end): sites = cells,
+// source weight mu_c vol / (4 pi d^2) exp(-plate path image -> cell), plate path
+// cell -> detector centre added to the attenuation at E', crystals attenuating
+// (cell, r) by the bounding-sphere criterion of scatter.cu:1960-2000 seen from
+// the cell (FP64); the rest is site_term.
+template <class R>
+static void cells_run(const float* col, const float* det, const float* img, const float* phy, double h, double keep,
+                      float* dst, const ScOpts& o) {
+    const int n = nfloor(det[0], 0.0001f);
+    const int X = nfloor(img[0], 0.001f), Y = nfloor(img[1], 0.001f), Z = nfloor(img[2], 0.001f);
+    const long V = (long)X * Y * Z;
+    const double fov2c = img[11];
+    std::vector<OPlate> P;
+    std::vector<OHole> H;
+    read_collimator(col, fov2c, P, H);
+    const std::vector<OCell> cells = make_cells(P, H, h, keep);
+    const int nc = (int)cells.size();
+    const int rows = o.det_end - o.det_begin;
+    const int nth = o.threads > 0 ? o.threads : omp_get_max_threads();
+    const bool use = nfloor(phy[0], 0.5f) == 1;
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nth)
+    for (int rr = 0; rr < rows; ++rr) {
+        const int r = o.det_begin + rr;
+        Row<R> D = make_row<R>(det, img, phy, r);
+        const float* dr = det + 12 * r;
+        const double Q[3] = {(double)dr[1], (double)dr[2] + fov2c, (double)dr[3]};
+        const double Rr = std::sqrt((double)dr[4] * dr[4] + (double)dr[6] * dr[6] + (double)dr[5] * dr[5]) / 2.0;
+        std::vector<std::vector<int>> lists(nc);
+        std::vector<double> out_pe(nc), out_c(nc);
+        for (int c = 0; c < nc; ++c) {
+            const double S[3] = {cells[c].x, cells[c].y, cells[c].z};
+            double pp[3];
+            plate_path(P, H, S, Q, pp);
+            out_pe[c] = pp[1];
+            out_c[c] = pp[2];
+            const double L = std::sqrt((Q[0] - S[0]) * (Q[0] - S[0]) + (Q[1] - S[1]) * (Q[1] - S[1]) +
+                                       (Q[2] - S[2]) * (Q[2] - S[2]));
+            for (int a = 0; a < n; ++a) {
+                if (a == r) continue;
+                const float* da = det + 12 * a;
+                const double K[3] = {(double)da[1], (double)da[2] + fov2c, (double)da[3]};
+                const double lk = std::sqrt((K[0] - S[0]) * (K[0] - S[0]) + (K[1] - S[1]) * (K[1] - S[1]) +
+                                            (K[2] - S[2]) * (K[2] - S[2]));
+                const double Rk = std::sqrt((double)da[4] * da[4] + (double)da[5] * da[5] + (double)da[6] * da[6]) / 2.0;
+                double e2 = 0.0;
+                for (int q = 0; q < 3; ++q) {
+                    const double e = (K[q] - S[q]) / lk - (Q[q] - S[q]) / L;
+                    e2 += e * e;
+                }
+                if (std::sqrt(e2) <= Rk / lk + Rr / L) lists[c].push_back(a);
+            }
+        }
+        for (long v = 0; v < V; ++v) {
+            R acc = R(0.0f);
+            if (use && D.mu_t > R(0.01f)) {
+                double xI, yI, zI;
+                voxel_point<double>(img, (int)v, true, xI, yI, zI);
+                const double I[3] = {xI, yI, zI};
+                for (int c = 0; c < nc; ++c) {
+                    const double S[3] = {cells[c].x, cells[c].y, cells[c].z};
+                    double pin[3];
+                    plate_path(P, H, I, S, pin);
+                    const double d2 = (S[0] - xI) * (S[0] - xI) + (S[1] - yI) * (S[1] - yI) + (S[2] - zI) * (S[2] - zI);
+                    const double w = cells[c].w / d2 * std::exp(-pin[0]);
+                    acc += site_term<R>(D, det, img, R(xI), R(yI), R(zI), R(S[0]), R(S[1]), R(S[2]), R(w), lists[c],
+                                        R(out_pe[c]), R(out_c[c]));
+                }
+            }
+            dst[(long)rr * V + v] = (float)to_double(acc);
+        }
+    }
+}
+
 }  // namespace oracle
 
 using oracle::ScOpts;
@@ -395,6 +606,37 @@ int or_scatter_collimator(const float* col, const float* det, const float* img, 
         oracle::collimator_run<double>(col, det, img, phy, bits, dst, *o);
     else
         oracle::collimator_run<float>(col, det, img, phy, bits, dst, *o);
+    return 0;
+}
+
+long long or_cells(const float* col, float fov2c, double h, double keep, double* out, long long cap) {
+    std::vector<oracle::OPlate> P;
+    std::vector<oracle::OHole> H;
+    oracle::read_collimator(col, fov2c, P, H);
+    const auto cells = oracle::make_cells(P, H, h, keep);
+    for (long long i = 0; i < (long long)cells.size() && i < cap; ++i) {
+        out[4 * i + 0] = cells[i].x;
+        out[4 * i + 1] = cells[i].y;
+        out[4 * i + 2] = cells[i].z;
+        out[4 * i + 3] = cells[i].w;
+    }
+    return (long long)cells.size();
+}
+
+int or_plate_path_f64(const float* col, float fov2c, const double* ab, double* out) {
+    std::vector<oracle::OPlate> P;
+    std::vector<oracle::OHole> H;
+    oracle::read_collimator(col, fov2c, P, H);
+    oracle::plate_path(P, H, ab, ab + 3, out);
+    return 0;
+}
+
+int or_scatter_cells(const float* col, const float* det, const float* img, const float* phy, double h, double keep,
+                     float* dst, const ScOpts* o, int f64) {
+    if (f64)
+        oracle::cells_run<double>(col, det, img, phy, h, keep, dst, *o);
+    else
+        oracle::cells_run<float>(col, det, img, phy, h, keep, dst, *o);
     return 0;
 }
 

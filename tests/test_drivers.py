@@ -116,3 +116,27 @@ def test_scatter_drivers_files(lib, tmp_path):
         assert np.array_equal(cry[rot], sc.scatter_crystal(g, pe[rot], rotation=rot)), f"crystal {rot}"
     tot = np.fromfile(tmp_path / shift_name("SysMat", g), np.float32).reshape(2, N, V)
     assert np.array_equal(tot, pe + cry)
+
+
+@pytest.mark.gpu
+def test_scatter_drivers_cells(lib, tmp_path):
+    """-cells SIZE KEEP: the collimator term comes from the plate-cell model."""
+    g = small_geometry()
+    g.write(str(tmp_path))
+    N, V = g.n_det, g.n_vox
+    pe = np.stack([oracle.pe(g, sub=(2, 2, 2), rot=r, precision="f64") for r in range(2)]).astype(np.float32)
+    pe.tofile(tmp_path / "pe.sysmat")
+    r = run("ScatterGen_Collimator", ["-PE", "pe.sysmat", "-cells", "1.0", "1.5"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    col = np.fromfile(tmp_path / shift_name("Scatter_SysMat", g), np.float32).reshape(2, N, V)
+    cells = [sc.scatter_cells(g, 1.0, 1.5, rotation=rot) for rot in range(2)]
+    for rot in range(2):
+        assert cells[rot].max() > 0
+        assert np.array_equal(col[rot], cells[rot]), f"cells rotation {rot}"
+    r = run("ScatterGen_CircularHole", ["-PE", "pe.sysmat", "-cells", "1.0", "1.5"], tmp_path)
+    assert r.returncode == 0, r.stdout + r.stderr
+    scat = np.fromfile(tmp_path / shift_name("Scatter_SysMat", g), np.float32).reshape(2, N, V)
+    for rot in range(2):
+        assert np.array_equal(scat[rot], cells[rot] + sc.scatter_crystal(g, pe[rot], rotation=rot)), f"rotation {rot}"
+    r = run("ScatterGen_Collimator", ["-PE", "pe.sysmat", "-cells", "0", "1"], tmp_path)
+    assert r.returncode != 0
