@@ -1,0 +1,98 @@
+"""Geometry checker (the reference's check_geometry_sanity.py role) and the dense
+.sysmat memmap reader (the layout its viewers read: det-major (N, Z, Y, X),
+visualize_sysmat_per_detector.py:77-90)."""
+import os
+import subprocess
+import sys
+
+import numpy as np
+import pytest
+
+from sysmat import geocheck, matrix, params
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+FIX = os.path.join(ROOT, "tests", "golden", "param_json")
+
+
+@pytest.mark.parametrize("name", ["c1", "c2", "c3", "c5"])
+def test_configs_are_clean(name):
+    rep = geocheck.check(params.CONFIGS[name]())
+    assert rep.errors == [] and rep.warnings == [], rep.text()
+
+
+def test_config4_variants_are_clean():
+    for g in params.config4_variants():
+        rep = geocheck.check(g)
+        assert rep.errors == [] and rep.warnings == [], rep.text()
+
+
+def test_large_detector_note():
+    rep = geocheck.check(params.config2())
+    assert any("exceed the reference drivers' fixed buffers" in s for s in rep.lines)
+
+
+def test_fixture_degenerate_holes_flagged():
+    """The shipped JSON example has y1 = y2 = 0 holes against a plate at -2.5 .. 2.5."""
+    col = np.fromfile(os.path.join(FIX, "Param_Collimator.dat"), np.float32)
+    det = np.fromfile(os.path.join(FIX, "Param_Detector.dat"), np.float32)
+    img = np.fromfile(os.path.join(FIX, "Param_Image.dat"), np.float32)
+    pad = lambda a, n: np.concatenate([a, np.zeros(max(0, n - a.size), np.float32)])
+    g = params.Geometry(pad(col, 100), pad(det, 13), pad(img, 100), np.zeros(100, np.float32))
+    rep = geocheck.check(g, det_file_floats=det.size)
+    assert rep.errors == []
+    assert any("y1 == y2" in w for w in rep.warnings)
+    assert any("do not span the plate" in w for w in rep.warnings)
+
+
+def test_problems_are_reported():
+    g = params.config3()
+    g.img[11] = 2.0                      # image centre 2 mm in front of the plate centre: FOV overlaps it
+    g.crystals[0, 1] = 0.0               # a crystal inside the plate
+    g.crystals[1, 6] = 0.0               # mu_total 0: a zero row
+    g.crystals[2, 7] = 0.5               # mu_pe + mu_c != mu_total
+    rep = geocheck.check(g)
+    text = " | ".join(rep.warnings)
+    assert "past the collimator front face" in text
+    assert "overlap plate 0" in text
+    assert "mu_total <= 0.01" in text
+    assert "mu_pe + mu_c != mu_total" in text
+    rep = geocheck.check(g, det_file_floats=100)
+    assert any("need" in e for e in rep.errors)
+
+
+def test_command_line(tmp_path):
+    g = params.config1()
+    g.write(str(tmp_path))
+    env = dict(os.environ, PYTHONPATH=os.path.join(ROOT, "gpu-spect-pet-sysmat_amd"))
+    args = [sys.executable, "-m", "sysmat.geocheck", "--param_collimator", str(tmp_path / "Params_Collimator.dat"),
+            "--param_detector", str(tmp_path / "Params_Detector.dat"), "--param_image", str(tmp_path / "Params_Image.dat")]
+    r = subprocess.run(args, capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 0 and r.stdout.strip().endswith("OK"), r.stdout + r.stderr
+    np.zeros(5, np.float32).tofile(tmp_path / "Params_Detector.dat")   # N = 0
+    r = subprocess.run(args, capture_output=True, text=True, env=env, timeout=120)
+    assert r.returncode == 1 and "ERROR" in r.stdout
+
+
+def test_sysmat_reader_layout(tmp_path):
+    g = params.tiny(seed=2, n_det_side=3, n_layers=1, image=(4, 3, 2))
+    g.img[6] = 2
+    g.img[8:11] = (0.5, -0.25, 1.0)
+    X, Y, Z = g.shape
+    N, V = g.n_det, g.n_vox
+    data = np.arange(2 * N * V, dtype=np.float32)
+    name = matrix.sysmat_name("PE_SysMat", g)
+    assert name == "PE_SysMat_shift_0.500000_-0.250000_1.000000.sysmat"
+    data.tofile(tmp_path / name)
+    m = matrix.open_for(g, str(tmp_path))
+    assert m.shape == (2, N, Z, Y, X)
+    rot, d, z, y, x = 1, 4, 1, 2, 3
+    assert m[rot, d, z, y, x] == ((rot * N + d) * V + z * X * Y + y * X + x)
+    np.testing.assert_array_equal(matrix.ppdf(m, d, rot), data[(rot * N + d) * V:(rot * N + d + 1) * V].reshape(Z, Y, X))
+    s = matrix.sensitivity(m, rot=0, block=4)
+    np.testing.assert_allclose(s, data[:N * V].reshape(N, Z, Y, X).astype(np.float64).sum(0))
+    a, b, c = matrix.mips(matrix.ppdf(m, d))
+    assert a.shape == (Y, X) and b.shape == (Z, X) and c.shape == (Z, Y)
+    vm = matrix.open_sysmat(str(tmp_path / name), N, (X, Y, Z), 2, order="vox_major")
+    assert vm.shape == (2, Z, Y, X, N)
+    with pytest.raises(ValueError):
+        matrix.open_sysmat(str(tmp_path / name), N + 1, (X, Y, Z), 2)
