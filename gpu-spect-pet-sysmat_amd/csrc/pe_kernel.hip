@@ -177,6 +177,23 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
     const float rho =
         0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
 
+    // ---- y-range of every sub-ray of the tile ------------------------------
+    // Sub-points of crystal d lie at y <= cy + (t/2)(1 - 1/n_y); voxel sub-points
+    // at y >= (voxel centre y) - pey.  When every sub-ray runs toward +y by at
+    // least 0.01 mm (so no sub-ray takes the y-parallel branches), a
+    // crystal whose y-range lies beyond the sub-points, or before the voxel
+    // sub-points, by more than the 1e-3 mm margin (>> FP32 rounding of the
+    // slab parameters at these distances) has a zero chord for every sub-ray:
+    // its y-slab interval starts after |PQ| or ends before 0
+    // (length_box_ray keeps only chords strictly inside the segment,
+    // PESysMatGen.cu:19-151).  Such crystals are dropped from the tile list.
+    const float hvx0 = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
+    const float hvy0 = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
+    const float pey0 = fabsf(p.rs) * hvx0 + fabsf(p.rc) * hvy0;
+    const float q_ymax = D.cy + D.hy * (1.0f - 1.0f / (float)p.cy) + 1e-3f;
+    const float tile_pymin = tcy - rho - pey0 - 1e-3f;
+    const bool tile_yclip = (D.cy - D.hy) - (tcy + rho + pey0) > 0.01f;
+
     // ---- workgroup cull: superset of every element's list ----------------
     // For any voxel centre I within rho of the tile centre I0, the reference
     // predicate |u_k - u_d| <= r_k/(2 D_k) + r_d/(2 D_d) implies
@@ -197,9 +214,12 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
             if (k < p.n_det && k != d) {
                 const float4 c0 = reinterpret_cast<const float4*>(cry + k)[0];
                 const float rb = cry[k].rbound;
+                const float khy = cry[k].hy;
                 const float kx = c0.x - tcx, ky = c0.y - tcy, kz = c0.z - tcz;
                 const float Dk = sqrtf(kx * kx + ky * ky + kz * kz);
-                if (Dk <= rho) {
+                if (tile_yclip && (c0.y - khy >= q_ymax || c0.y + khy <= tile_pymin)) {
+                    take = false;
+                } else if (Dk <= rho) {
                     take = true;
                 } else {
                     const float iD = 1.0f / Dk;
