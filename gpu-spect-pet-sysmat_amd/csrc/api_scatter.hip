@@ -79,9 +79,88 @@ struct ScatterGeom {
     DeviceBuf<Hole> hol;
     DeviceBuf<ScRow> rows;
     std::vector<ScRow> hrows;
+    DeviceBuf<SlabGrid> grids;
+    DeviceBuf<int> grid_cell, grid_item;
     ScParams base{};
     int n_holes0 = 0;
 };
+
+// Crystals of each y-slab binned by centre on an x-z grid whose cell is the
+// slab's largest crystal footprint (doubled until the grid has at most
+// 4 x members + 65536 cells).  Members of a cell stay in index order.
+static int build_grids(ScatterGeom& G, double fov2c) {
+    const std::vector<Crystal>& C = G.T.crystals;
+    if (G.T.slabs.empty()) return SM_OK;  // too many distinct y-ranges: scan every crystal
+    int max_slab = -1;
+    for (const Crystal& c : C) max_slab = std::max(max_slab, (int)c.slab);
+    std::vector<SlabGrid> grids;
+    std::vector<int> cell, item;
+    for (int sid = 0; sid <= max_slab; ++sid) {
+        std::vector<int> mem;
+        for (int k = 0; k < (int)C.size(); ++k)
+            if ((int)C[k].slab == sid) mem.push_back(k);
+        if (mem.empty()) continue;
+        SlabGrid g{};
+        double xl = INFINITY, xh = -INFINITY, zl = INFINITY, zh = -INFINITY, cs = 0.0, m = 0.0;
+        g.y1 = INFINITY;
+        g.y2 = -INFINITY;
+        for (int k : mem) {
+            const Crystal& c = C[k];
+            const double yc = (double)c.y_raw + fov2c;
+            g.y1 = std::min(g.y1, yc - (double)c.hy);
+            g.y2 = std::max(g.y2, yc + (double)c.hy);
+            xl = std::min(xl, (double)c.cx);
+            xh = std::max(xh, (double)c.cx);
+            zl = std::min(zl, (double)c.cz);
+            zh = std::max(zh, (double)c.cz);
+            const double e = (c.rot == 0.0f) ? std::max((double)c.hx, (double)c.hz)
+                                             : std::sqrt((double)c.hx * c.hx + (double)c.hz * c.hz);
+            m = std::max(m, e);
+            cs = std::max(cs, 2.0 * e);
+        }
+        if (!(cs > 0.0)) cs = 1.0;
+        long long nx, nz;
+        for (;;) {
+            nx = (long long)std::floor((xh - xl) / cs) + 2;
+            nz = (long long)std::floor((zh - zl) / cs) + 2;
+            if (nx * nz <= 4LL * (long long)mem.size() + 65536) break;
+            cs *= 2.0;
+        }
+        g.cx = g.cz = cs;
+        g.x0 = xl - 0.5 * cs;
+        g.z0 = zl - 0.5 * cs;
+        g.nx = (int)nx;
+        g.nz = (int)nz;
+        g.m = m + 0.01;
+        g.cell_base = (int)cell.size();
+        std::vector<int> cnt((size_t)(nx * nz) + 1, 0);
+        std::vector<int> where(mem.size());
+        for (size_t i = 0; i < mem.size(); ++i) {
+            const Crystal& c = C[mem[i]];
+            const long long ix = std::min(nx - 1, std::max(0LL, (long long)std::floor(((double)c.cx - g.x0) / cs)));
+            const long long iz = std::min(nz - 1, std::max(0LL, (long long)std::floor(((double)c.cz - g.z0) / cs)));
+            where[i] = (int)(iz * nx + ix);
+            ++cnt[(size_t)where[i] + 1];
+        }
+        for (size_t c = 1; c < cnt.size(); ++c) cnt[c] += cnt[c - 1];
+        const int item0 = (int)item.size();
+        item.resize(item.size() + mem.size());
+        std::vector<int> fill(cnt.begin(), cnt.end() - 1);
+        for (size_t i = 0; i < mem.size(); ++i) item[(size_t)item0 + fill[(size_t)where[i]]++] = mem[i];
+        for (int& c : cnt) c += item0;
+        cell.insert(cell.end(), cnt.begin(), cnt.end());
+        grids.push_back(g);
+    }
+    int rc;
+    if ((rc = G.grids.upload(grids)) < 0) return rc;
+    if ((rc = G.grid_cell.upload(cell)) < 0) return rc;
+    if ((rc = G.grid_item.upload(item)) < 0) return rc;
+    G.base.grids = G.grids.ptr;
+    G.base.grid_cell = G.grid_cell.ptr;
+    G.base.grid_item = G.grid_item.ptr;
+    G.base.n_grids = (int)grids.size();
+    return SM_OK;
+}
 
 static int build_geom(const float* col, const float* det, const float* img, const float* phy, int device,
                       int rotation, ScatterGeom& G) {
@@ -142,7 +221,9 @@ static int build_geom(const float* col, const float* det, const float* img, cons
         window_band(p.e0, R.res, R.lo, R.hi, R.th_lo, R.th_hi);
         R.use = (use_compton && D[7] > 0.01f && D[8] != 0.0f) ? 1.0f : 0.0f;
     }
-    return G.rows.upload(G.hrows);
+    int rc2 = G.rows.upload(G.hrows);
+    if (rc2 < 0) return rc2;
+    return build_grids(G, (double)img[11]);
 }
 
 long long crystal_words(int n) { return ((long long)n * (n + 1) / 2 * n + 31) / 32; }
@@ -232,11 +313,12 @@ static int run_blocks(const ScatterGeom& G, ScParams p, const unsigned* bits_dev
     const long long pair_budget = 2048ll << 20;  // 2 GiB of pair records per block
     int block = (int)std::max(4ll, std::min<long long>(det_count, pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
     block = (block + 3) / 4 * 4;
+    p.site_chunk = kPrepChunk;
     DeviceBuf<ScPair> pairs;
     DeviceBuf<int> counts;
     int rc;
     if ((rc = pairs.alloc((size_t)block * p.max_sites)) < 0) return rc;
-    if ((rc = counts.alloc(block)) < 0) return rc;
+    if ((rc = counts.alloc((size_t)block * site_chunks(p))) < 0) return rc;
     for (int b0 = 0; b0 < det_count; b0 += block) {
         const int cnt = std::min(block, det_count - b0);
         p.det_begin = det_begin + b0;
@@ -270,12 +352,13 @@ static int run_scatter_list(ScatterGeom& G, int mode, const unsigned* bits, cons
     const long long pair_budget = 2048ll << 20;
     const int chunk = (int)std::max(1ll, std::min<long long>((long long)rows.size(),
                                                              pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
+    p.site_chunk = kPrepChunk;
     DeviceBuf<ScPair> pairs;
     DeviceBuf<int> counts, drows;
     DeviceBuf<int2> ditems;
     int rc;
     if ((rc = pairs.alloc((size_t)chunk * p.max_sites)) < 0) return rc;
-    if ((rc = counts.alloc(chunk)) < 0) return rc;
+    if ((rc = counts.alloc((size_t)chunk * site_chunks(p))) < 0) return rc;
     if ((rc = drows.alloc(chunk)) < 0) return rc;
     std::vector<int2> local;
     for (size_t c0 = 0; c0 < rows.size(); c0 += chunk) {
@@ -1102,6 +1185,7 @@ int batch_enqueue(SmScatterBatch& B, const float* const* pe, float* const* dst, 
                 ScParams p = G.base;
                 p.mode = term == 0 ? 1 : 0;
                 p.max_sites = term == 0 ? std::max(1, G.n_holes0) : n;
+                p.site_chunk = kPrepChunk;
                 p.det_begin = b0;
                 p.det_count = cnt;
                 p.row_stride = p.V;
@@ -1151,7 +1235,8 @@ int sm_scatter_batch_create(int n_variants, const float* const* cols, const floa
     B->block = (int)std::max(4ll, std::min<long long>(B->G[0]->T.n_det, pair_budget / (max_sites * (long long)sizeof(ScPair))));
     B->block = (B->block + 3) / 4 * 4;
     if ((rc = B->pairs.alloc((size_t)B->block * max_sites)) < 0) return fail(rc);
-    if ((rc = B->counts.alloc(B->block)) < 0) return fail(rc);
+    if ((rc = B->counts.alloc((size_t)B->block * (size_t)((max_sites + kPrepChunk - 1) / kPrepChunk))) < 0)
+        return fail(rc);
     *out = B;
     return SM_OK;
 }

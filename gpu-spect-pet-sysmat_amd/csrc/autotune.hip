@@ -1,8 +1,8 @@
 // Tile-shape autotuner of the PE tracer: times candidate voxel tiles on a
 // sample of detector rows spread over the detector and keeps the fastest.
 // Results persist in a small text cache keyed by a hash of the geometry, the
-// image, the sampling and the GPU architecture, so a configuration is tuned
-// once per machine type.
+// image, the sampling, the GPU architecture and the tracer's schedule revision,
+// so a configuration is tuned once per machine type and kernel version.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,6 +15,7 @@
 
 #include "../../include/sysmat.h"
 #include "common.hpp"
+#include "pe_kernel.hpp"
 
 using namespace sysmat;
 
@@ -41,6 +42,7 @@ std::string tune_key(const float* col, const float* det, const float* img, const
     const int samp[4] = {o.voxel_sub, o.sub_x, o.sub_y, o.sub_z};
     h = fnv1a(samp, sizeof(samp), h);
     h = fnv1a(arch, std::strlen(arch), h);
+    h = fnv1a(&kPeScheduleRevision, sizeof(kPeScheduleRevision), h);
     char buf[64];
     std::snprintf(buf, sizeof(buf), "%016llx", (unsigned long long)h);
     return buf;

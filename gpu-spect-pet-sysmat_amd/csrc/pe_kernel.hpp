@@ -8,6 +8,9 @@ namespace sysmat {
 
 constexpr int kCapCrystals = 4096;  // tile candidate crystals (reference per-element cap: 3000, PESysMatGen.cu:481)
 constexpr int kCapHoles = 1024;     // tile candidate holes (reference: 1000 per layer, PESysMatGen.cu:485)
+// Revision of the tracer's schedule; part of the autotuner's cache key, so tile
+// choices tuned for an older schedule are not reused.
+constexpr int kPeScheduleRevision = 2;
 
 struct PeParams {
     // image slab (in-band Img[2], Img[30], Img[31] of the reference ABI)

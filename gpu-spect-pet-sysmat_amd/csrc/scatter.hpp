@@ -20,6 +20,22 @@
 
 namespace sysmat {
 
+// Crystals of one y-slab (detector layer) binned by centre on a uniform x-z grid,
+// for finding the crystals a segment can cross without scanning all N.
+struct alignas(16) SlabGrid {
+    double y1, y2;            // slab y-range (FOV2C added)
+    double x0, z0;            // grid origin
+    double cx, cz;            // cell size
+    double m;                 // largest x-z half-extent of the slab's crystals, plus a margin
+    int nx, nz;               // cells
+    int cell_base;            // first entry of this grid in the cell offset table
+    int pad;
+};
+
+// Pair-table chunk: the preparation kernel handles (row, chunk of kPrepChunk
+// sites) per workgroup and compacts survivors within the chunk.
+constexpr int kPrepChunk = 256;
+
 // Per detecting crystal r (FP64 where the value feeds a branch).
 struct alignas(16) ScRow {
     double cx, cy, cz;        // crystal centre (sub-point, divide 1x1x1), FOV2C added in FP64
@@ -73,15 +89,31 @@ struct ScParams {
     const int* row_list;      // prep / accumulate rows: r = row_list[rr], else det_begin + rr
     int dst_by_row;           // with row_list: 1 = dst row r - det_begin, 0 = dst row rr
     int site_cap;             // accumulate over the first site_cap pairs of each row only
+    int site_chunk;           // pairs of a row come in chunks of site_chunk (counts per chunk);
+                              // 0: one compacted list per row (counts per row)
     const int2* items;        // list mode: wave w of block b handles items[4b + w] = (rr, voxel tile)
     long long n_items;
     int accumulate;           // 1: dst += element (e.g. crystal scatter onto the collimator term)
+    // crystal grids per y-slab (n_grids == 0: attenuators found by scanning every crystal)
+    const SlabGrid* grids;
+    const int* grid_cell;     // cell -> first item; cell c of grid g spans [grid_cell[b + c], grid_cell[b + c + 1]),
+                              // b = cell_base of g (one extra entry per grid)
+    const int* grid_item;     // crystal indices, ascending within a cell
+    int n_grids;
     // collimator cells (mode 3)
     const Cell* cells;        // this launch's cells; pair.site indexes them and the weight table rows
     const CellPlate* cplates;
     const CellHole* choles;
     int n_cplates;
 };
+
+// Chunks of a row's pair table and their size.
+__host__ __device__ inline int site_chunks(const ScParams& p) {
+    return p.site_chunk > 0 ? (p.max_sites + p.site_chunk - 1) / p.site_chunk : 1;
+}
+__host__ __device__ inline int site_chunk_size(const ScParams& p) {
+    return p.site_chunk > 0 ? p.site_chunk : p.max_sites;
+}
 
 // Scatter-surrogate near field: per detector the n_near + n_ring sites with the
 // largest single-site weight at the image centre, best first; pair.pad holds the

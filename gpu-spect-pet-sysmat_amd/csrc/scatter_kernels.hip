@@ -182,6 +182,60 @@ __device__ inline ImageSphere image_sphere(const ScParams& p) {
     return b;
 }
 
+// Every crystal whose box the segment A -> B can cross (a superset; others have a
+// zero chord): per slab grid, the segment clipped to the slab's y-range, then the
+// grid rows its z-extent covers and, per row, the columns its x-extent covers,
+// all widened by the grid's margin.  Visits slabs, rows, columns and cell
+// members in ascending order, each crystal once.
+template <class F>
+__device__ inline void segment_crystals(const ScParams& p, double ax, double ay, double az, double bx, double by,
+                                        double bz, F&& visit) {
+    const double dx = bx - ax, dy = by - ay, dz = bz - az;
+    for (int g = 0; g < p.n_grids; ++g) {
+        const SlabGrid G = p.grids[g];
+        double t0 = 0.0, t1 = 1.0;
+        const double ylo = G.y1 - 1e-3, yhi = G.y2 + 1e-3;
+        if (dy != 0.0) {
+            double ta = (ylo - ay) / dy, tb = (yhi - ay) / dy;
+            if (ta > tb) { const double q = ta; ta = tb; tb = q; }
+            t0 = fmax(t0, ta);
+            t1 = fmin(t1, tb);
+        } else if (ay < ylo || ay > yhi) {
+            continue;
+        }
+        if (!(t1 >= t0)) continue;
+        const double xa = ax + t0 * dx, za = az + t0 * dz, xb = ax + t1 * dx, zb = az + t1 * dz;
+        const double m = G.m, icx = 1.0 / G.cx, icz = 1.0 / G.cz;
+        const double xmin = fmin(xa, xb) - m, xmax = fmax(xa, xb) + m;
+        const double zmin = fmin(za, zb) - m, zmax = fmax(za, zb) + m;
+        const int ix0 = max(0, (int)floor((xmin - G.x0) * icx)), ix1 = min(G.nx - 1, (int)floor((xmax - G.x0) * icx));
+        const int iz0 = max(0, (int)floor((zmin - G.z0) * icz)), iz1 = min(G.nz - 1, (int)floor((zmax - G.z0) * icz));
+        if (ix0 > ix1 || iz0 > iz1) continue;
+        const double ddz = zb - za, ddx = xb - xa;
+        for (int iz = iz0; iz <= iz1; ++iz) {
+            double u0 = xmin, u1 = xmax;
+            if (ddz != 0.0) {
+                // part of the clipped segment whose z lies within the row's widened window
+                const double w0 = G.z0 + (double)iz * G.cz - m, w1 = G.z0 + (double)(iz + 1) * G.cz + m;
+                double s0 = (w0 - za) / ddz, s1 = (w1 - za) / ddz;
+                if (s0 > s1) { const double q = s0; s0 = s1; s1 = q; }
+                s0 = fmax(s0, 0.0);
+                s1 = fmin(s1, 1.0);
+                if (s0 > s1) continue;
+                const double x0 = xa + s0 * ddx, x1 = xa + s1 * ddx;
+                u0 = fmin(x0, x1) - m;
+                u1 = fmax(x0, x1) + m;
+            }
+            const int jx0 = max(ix0, (int)floor((u0 - G.x0) * icx)), jx1 = min(ix1, (int)floor((u1 - G.x0) * icx));
+            const int* cellp = p.grid_cell + G.cell_base + (long long)iz * G.nx;
+            for (int ix = jx0; ix <= jx1; ++ix) {
+                const int e1 = cellp[ix + 1];
+                for (int e = cellp[ix]; e < e1; ++e) visit(p.grid_item[e]);
+            }
+        }
+    }
+}
+
 // Per-(detector r, site s) constants.  Returns false when the pair cannot
 // contribute for any voxel (dead site, or scatter angle outside the window band
 // over the whole image).  Attenuators come from the bitmap when `bits` is given,
@@ -238,57 +292,53 @@ __device__ inline bool make_pair(const ScParams& p, const ScRow& R, int r, int s
     P.l_abs = (float)chord_box64(ssx, ssy, ssz, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
     const double q = R.rdet / L;
     P.c_fwd = (q >= 1.0) ? -2.0f : (float)(sqrt(1.0 - q * q) - 1e-6);
-    // attenuators (scatter.cu:1343-1401 / 1839-1897)
+    // attenuators (scatter.cu:1343-1401 / 1839-1897): crystals that pass the
+    // GeometryRelationShip criterion and have a non-zero chord on S -> Q.  With
+    // slab grids only crystals the segment can cross are examined (every other
+    // crystal has a zero chord), else all of them.
     double ape = 0.0, ac = 0.0;
-    if (p.mode == 3) {
-        // plate path to the detector centre, then the crystals passing the
-        // bounding-sphere criterion seen from the cell (FP64)
+    if (p.mode == 3) {  // plate path of a cell site to the detector centre
         const PathAtt o = plate_path(p.cplates, p.n_cplates, p.choles, sx, sy, sz, R.cx, R.cy, R.cz);
         ape = o.pe;
         ac = o.c;
-        const double ujx = bx / L, ujy = by / L, ujz = bz / L, cj = R.rdet / L;
-        for (int a = 0; a < n; ++a) {
-            if (a == r) continue;
-            const Crystal A = cry[a];
+    }
+    const int i0 = s < r ? s : r, j0 = s < r ? r : s;
+    const long long pair_bit = ((long long)i0 * n - ((long long)i0 * (i0 + 1)) / 2 + j0) * n;
+    CrossIJ cij{};
+    CrossHJ chj{};
+    if (!bits && p.mode == 0) cij = cross_ij(cry[i0], cry[j0]);
+    if (!bits && (p.mode == 1 || p.mode == 2)) chj = cross_hj(hol[s], cry[r]);
+    const double ujx = bx / L, ujy = by / L, ujz = bz / L, cj = R.rdet / L;  // cell criterion (FP64)
+    auto criterion = [&](int a, const Crystal& A) -> bool {
+        if (p.mode == 0) return bits ? test_bit(bits, pair_bit + a) : cross_k(cij, A);
+        if (p.mode == 3) {
             const double kx = (double)A.cx - sx, ky = ycen(A, p.fov2c) - sy, kz = (double)A.cz - sz;
             const double lk = sqrt(kx * kx + ky * ky + kz * kz);
             const double hx = A.hx, hy = A.hy, hz = A.hz;
             const double ck = sqrt(hx * hx + hy * hy + hz * hz) / lk;
             const double ex = kx / lk - ujx, ey = ky / lk - ujy, ez = kz / lk - ujz;
-            if (!(sqrt(ex * ex + ey * ey + ez * ez) <= ck + cj)) continue;
-            const double c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
-            ape += c * (double)A.mu_pe;
-            ac += c * (double)A.mu_c;
+            return sqrt(ex * ex + ey * ey + ez * ez) <= ck + cj;
         }
-        P.A_pe = (float)ape;
-        P.A_c = (float)ac;
-        return true;
-    }
-    const int i0 = s < r ? s : r, j0 = s < r ? r : s;
-    CrossIJ cij{};
-    CrossHJ chj{};
-    if (!bits) {
-        if (p.mode == 0) cij = cross_ij(cry[i0], cry[j0]);
-        else chj = cross_hj(hol[s], cry[r]);
-    }
-    for (int a = 0; a < n; ++a) {
-        if (a == r) continue;
-        bool hit;
-        if (p.mode == 0) {
-            if (a == s) continue;
-            if (bits) {
-                hit = test_bit(bits, ((long long)i0 * n - ((long long)i0 * (i0 + 1)) / 2 + j0) * n + a);
-            } else {
-                hit = cross_k(cij, cry[a]);
-            }
-        } else {
-            hit = bits ? test_bit(bits, (long long)s * n * n + (long long)r * n + a) : cross_hk(chj, cry[a]);
-        }
-        if (!hit) continue;
+        return bits ? test_bit(bits, (long long)s * n * n + (long long)r * n + a) : cross_hk(chj, A);
+    };
+    auto visit = [&](int a) {
+        if (a == r || (p.mode == 0 && a == s)) return;
         const Crystal A = cry[a];
-        const double c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
+        double c;
+        if (bits) {  // cheap test first
+            if (!criterion(a, A)) return;
+            c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
+        } else {
+            c = att_chord(A, p.fov2c, sx, sy, sz, R.cx, R.cy, R.cz);
+            if (c == 0.0 || !criterion(a, A)) return;
+        }
         ape += c * (double)A.mu_pe;
         ac += c * (double)A.mu_c;
+    };
+    if (p.n_grids > 0) {
+        segment_crystals(p, sx, sy, sz, R.cx, R.cy, R.cz, visit);
+    } else {
+        for (int a = 0; a < n; ++a) visit(a);
     }
     P.A_pe = (float)ape;
     P.A_c = (float)ac;
@@ -297,15 +347,18 @@ __device__ inline bool make_pair(const ScParams& p, const ScRow& R, int r, int s
 
 __device__ inline int prep_row(const ScParams& p, int rr) { return p.row_list ? p.row_list[rr] : p.det_begin + rr; }
 
-// One workgroup per detecting crystal r; lanes evaluate sites, survivors are
-// compacted in index order into pairs[rr * max_sites + slot].
+// One workgroup per (detector row, chunk of kPrepChunk sites); lanes evaluate
+// sites, survivors are compacted in index order into
+// pairs[rr * max_sites + chunk * kPrepChunk + slot], their number into
+// counts[rr * chunks + chunk].
 __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScRow* __restrict__ rows,
                                                            const Crystal* __restrict__ cry,
                                                            const Hole* __restrict__ hol,
                                                            const unsigned* __restrict__ bits,
                                                            ScPair* __restrict__ pairs, int* __restrict__ counts) {
     __shared__ int s_wcount[4];
-    const int rr = blockIdx.x;
+    const int nch = site_chunks(p);
+    const int rr = (int)(blockIdx.x / nch), ch = (int)(blockIdx.x % nch);
     const int r = prep_row(p, rr);
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
@@ -314,25 +367,21 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
     const ImageSphere B = image_sphere(p);
     int count = 0;
     if (R.use != 0.0f) {
-        for (int base = 0; base < n_sites; base += 256) {
-            const int s = base + tid;
-            ScPair P;
-            const bool alive = s < n_sites && !(p.mode == 0 && s == r) && make_pair(p, R, r, s, cry, hol, bits, B, P);
-            const unsigned long long bal = __ballot(alive);
-            if (lane == 0) s_wcount[wave] = __popcll(bal);
-            __syncthreads();
-            int off = count, all = 0;
-            for (int w2 = 0; w2 < 4; ++w2) {
-                const int c = s_wcount[w2];
-                if (w2 < wave) off += c;
-                all += c;
-            }
-            if (alive) pairs[(long long)rr * p.max_sites + off + __popcll(bal & below)] = P;
-            count += all;
-            __syncthreads();
+        const int s = ch * kPrepChunk + tid;
+        ScPair P;
+        const bool alive = s < n_sites && !(p.mode == 0 && s == r) && make_pair(p, R, r, s, cry, hol, bits, B, P);
+        const unsigned long long bal = __ballot(alive);
+        if (lane == 0) s_wcount[wave] = __popcll(bal);
+        __syncthreads();
+        int off = 0;
+        for (int w2 = 0; w2 < 4; ++w2) {
+            const int c = s_wcount[w2];
+            if (w2 < wave) off += c;
+            count += c;
         }
+        if (alive) pairs[(long long)rr * p.max_sites + (long long)ch * kPrepChunk + off + __popcll(bal & below)] = P;
     }
-    if (tid == 0) counts[rr] = count;
+    if (tid == 0) counts[(long long)rr * nch + ch] = count;
 }
 
 // ---------------------------------------------------------------------------
@@ -482,14 +531,19 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
     if (R.use != 0.0f && active) {
         const VoxelPt q = voxel_of(p, R, v);
         const KnCoef kc = kn_coef(p.e0);
-        int n_sites = counts[rr];
-        if (p.site_cap > 0 && n_sites > p.site_cap) n_sites = p.site_cap;
-        const ScPair* P = pairs + (long long)rr * p.max_sites;
-        for (int j = 0; j < n_sites; ++j) {
-            const ScPair S = P[j];
-            const bool table = p.mode == 0 || p.mode == 3;
-            const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
-            acc += site_term(p, R, S, q, kc, w);
+        const int nch = site_chunks(p), csz = site_chunk_size(p);
+        int cap = (p.site_cap > 0) ? p.site_cap : 0x7fffffff;
+        const bool table = p.mode == 0 || p.mode == 3;
+        for (int c = 0; c < nch && cap > 0; ++c) {
+            int n_c = counts[(long long)rr * nch + c];
+            if (n_c > cap) n_c = cap;
+            cap -= n_c;
+            const ScPair* P = pairs + (long long)rr * p.max_sites + (long long)c * csz;
+            for (int j = 0; j < n_c; ++j) {
+                const ScPair S = P[j];
+                const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
+                acc += site_term(p, R, S, q, kc, w);
+            }
         }
     }
     if (active) {
@@ -622,7 +676,9 @@ hipError_t launch_bitmap_collimator(const Hole* hol, int n_holes, const Crystal*
 hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Crystal* cry, const Hole* hol,
                                const unsigned* bits, ScPair* pairs, int* counts, hipStream_t stream) {
     if (p.det_count <= 0) return hipSuccess;
-    hipLaunchKernelGGL(scatter_prep_kernel, dim3((unsigned)p.det_count), dim3(256), 0, stream, p, rows, cry, hol, bits,
+    if (p.site_chunk != kPrepChunk) return hipErrorInvalidValue;  // pair tables come in kPrepChunk chunks
+    const long long blocks = (long long)p.det_count * site_chunks(p);
+    hipLaunchKernelGGL(scatter_prep_kernel, dim3((unsigned)blocks), dim3(256), 0, stream, p, rows, cry, hol, bits,
                        pairs, counts);
     return hipGetLastError();
 }
@@ -659,7 +715,7 @@ hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pa
         const bool shift_dst = !(p.row_list && p.dst_by_row);
         const long long grid = (q.n_blocks + 7) / 8 * 8;
         hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)grid), dim3(256), 0, stream, q, rows,
-                           pairs + (long long)d0 * p.max_sites, counts + d0, pe,
+                           pairs + (long long)d0 * p.max_sites, counts + (long long)d0 * site_chunks(p), pe,
                            shift_dst ? dst + (long long)d0 * p.row_stride : dst);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
