@@ -199,6 +199,17 @@ int sm_csr_copy(const SmCsr* c, long long* row_ptr, int* col_idx, float* val);
 int sm_csr_write(const char* path, const SmCsr* const* parts, int n_parts);
 int sm_csr_destroy(SmCsr* c);
 
+/* Beam masks and multiplexing index of device CSR rows (row_ptr rows+1 int64,
+ * col int32, val float32, as from sm_csr_count_device / sm_csr_fill_device).  The
+ * reference computes these from dense .sysmat files on the host
+ * (PE/extract_beam_prop_masks.py, PE/compute_mpxi_3d.py); here they read the CSR
+ * output on the GPU.  An element is in its row's beam when (double)v > frac *
+ * (double)(row max); mpxi_dev (uint32, one per column) gains 1 per element in a
+ * beam (it accumulates: clear it first); beam_size_dev (int32 per row, nullable)
+ * receives each row's beam size.  Asynchronous on `stream`. */
+int sm_csr_beams_device(const long long* row_ptr_dev, const int* col_dev, const float* val_dev, int rows,
+                        double frac, unsigned* mpxi_dev, int* beam_size_dev, void* stream);
+
 /* ---- scatter-surrogate MLP (no counterpart in the reference) ------------
  * 12 -> 256 -> 256 -> 256 -> 256 -> 1, SiLU; bf16 operands on the matrix cores,
  * FP32 accumulation.  Parameters are sm_mlp_param_count() FP32 values in device
