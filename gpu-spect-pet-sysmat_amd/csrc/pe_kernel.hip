@@ -50,10 +50,9 @@ __device__ inline double wave_sum64(double v) {
     return v;
 }
 
-template <class T>
-__device__ inline T uniform(T v) {
-    return __builtin_amdgcn_readfirstlane(v);
-}
+// Wave-uniform copy of an integer (bit-preserving for unsigned).
+__device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+__device__ inline unsigned uniform(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
 
 // q = s / d, r = s % d for 0 <= s < 2^24 via a float reciprocal and one correction.
 __device__ inline int divmod(int s, int d, float rd, int& r) {
@@ -573,7 +572,9 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
                 }
                 // other crystals on the way (PESysMatGen.cu:687-725)
                 if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
-                    // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the wave
+                    // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
+                    // wave; unrolled so the LDS reads of several boxes are in flight together
+#pragma unroll 4
                     for (int j = 0; j < n_att; ++j) {
                         const Att A = s_att[wave][j];
                         att += chord_world(R, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
