@@ -81,6 +81,7 @@ struct ScatterGeom {
     std::vector<ScRow> hrows;
     DeviceBuf<SlabGrid> grids;
     DeviceBuf<int> grid_cell, grid_item;
+    DeviceBuf<double2> kn_tab;
     ScParams base{};
     int n_holes0 = 0;
 };
@@ -185,6 +186,16 @@ static int build_geom(const float* col, const float* det, const float* img, cons
     if (!(p.e0 > 0.0)) return set_error(SM_E_ARG, "photopeak energy Phys[7] must be positive");
     p.kn_total = kn_total_ref(p.e0);
     p.h = (double)0.01f;
+    {
+        std::vector<double2> tab(kKnSteps + 1);
+        for (int n = 0; n <= kKnSteps; ++n) {
+            const double a = (double)n * p.h;
+            tab[n] = make_double2(std::cos(a), std::sin(a));
+        }
+        int rc0 = G.kn_tab.upload(tab);
+        if (rc0 < 0) return rc0;
+        p.kn_tab = G.kn_tab.ptr;
+    }
     p.n_det = G.T.n_det;
     p.V = (long long)p.X * p.Y * p.Z;
     p.fov2c = img[11];

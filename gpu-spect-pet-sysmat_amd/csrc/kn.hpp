@@ -13,6 +13,9 @@
 //     H(w) = -1/(2 w^2) + ln w + 1/w + (w - beta^2/w - 2 beta ln w) / alpha^2,  beta = 1 + alpha.
 // The neglected term is below 1e-10 of S for h = 0.01; the step count n is
 // taken exactly as the reference does, so the discrete structure is kept.
+// The caller passes cos / sin of the start angle (the scatter kernel has cos
+// theta_min itself) and the tabulated (cos n h, sin n h), so no trigonometric
+// function is evaluated per site.
 #pragma once
 #include <cmath>
 
@@ -37,14 +40,8 @@ KN_HD inline KnCoef kn_coef(double e0) {
     return c;
 }
 
-KN_HD inline double kn_H(const KnCoef& c, double w) {
-    const double lw = log(w), iw = 1.0 / w;
-    return -0.5 * iw * iw + lw + iw + (w - c.beta * c.beta * iw - 2.0 * c.beta * lw) * c.ia2;
-}
-
-// f(theta) and f'(theta) at cos = u, sin = s.
-KN_HD inline void kn_f(const KnCoef& c, double u, double s, double& f, double& df) {
-    const double P = 1.0 / (1.0 + c.alpha * (1.0 - u));
+// f(theta) and f'(theta) at cos = u, sin = s, with P = 1 / (1 + alpha (1 - u)).
+KN_HD inline void kn_fP(const KnCoef& c, double u, double s, double P, double& f, double& df) {
     const double P2 = P * P, P3 = P2 * P;
     const double g = P3 + P - P2 + u * u * P2;
     const double dg = c.alpha * (3.0 * P2 * P2 + P2 - 2.0 * P3) + 2.0 * u * P2 + 2.0 * c.alpha * u * u * P3;
@@ -52,16 +49,24 @@ KN_HD inline void kn_f(const KnCoef& c, double u, double s, double& f, double& d
     df = -dg * s * s + g * u;
 }
 
-// h * sum_{i=0..n} f(a + i h)
-KN_HD inline double kn_sum(const KnCoef& c, double a, int n, double h) {
-    const double b = a + (double)n * h;
-    const double ua = cos(a), sa = sin(a), ub = cos(b), sb = sin(b);
-    double fa, dfa, fb, dfb;
-    kn_f(c, ua, sa, fa, dfa);
-    kn_f(c, ub, sb, fb, dfb);
+// h * sum_{i=0..n} f(a + i h) from cos / sin of a and (cos n h, sin n h): the end
+// angle b = a + n h by angle addition, the antiderivative difference through one
+// log1p,
+//   H(wb) - H(wa) = -(1/wb^2 - 1/wa^2)/2 + ln(wb/wa) + (1/wb - 1/wa)
+//                   + ((wb - wa) - beta^2 (1/wb - 1/wa) - 2 beta ln(wb/wa)) / alpha^2,
+// with wb - wa = alpha (cos a - cos b) taken directly (no cancellation).
+KN_HD inline double kn_sum_cs(const KnCoef& c, double ua, double sa, double cnh, double snh, double h) {
+    const double ub = ua * cnh - sa * snh, sb = sa * cnh + ua * snh;
     const double wa = 1.0 + c.alpha * (1.0 - ua), wb = 1.0 + c.alpha * (1.0 - ub);
-    const double integral = (kn_H(c, wb) - kn_H(c, wa)) * c.ia;
-    return integral + 0.5 * h * (fa + fb) + h * h / 12.0 * (dfb - dfa);
+    const double iwa = 1.0 / wa, iwb = 1.0 / wb;
+    double fa, dfa, fb, dfb;
+    kn_fP(c, ua, sa, iwa, fa, dfa);
+    kn_fP(c, ub, sb, iwb, fb, dfb);
+    const double dw = c.alpha * (ua - ub);
+    const double diw = -dw * iwa * iwb;
+    const double dl = log1p(dw * iwa);
+    const double dH = -0.5 * diw * (iwb + iwa) + dl + diw + (dw - c.beta * c.beta * diw - 2.0 * c.beta * dl) * c.ia2;
+    return dH * c.ia + 0.5 * h * (fa + fb) + h * h / 12.0 * (dfb - dfa);
 }
 
 }  // namespace sysmat

@@ -32,6 +32,9 @@ struct alignas(16) SlabGrid {
     int pad;
 };
 
+// Riemann steps of the KN weight never exceed pi / h = 314 (h = 0.01f).
+constexpr int kKnSteps = 320;
+
 // Pair-table chunk: the preparation kernel handles (row, chunk of kPrepChunk
 // sites) per workgroup and compacts survivors within the chunk.
 constexpr int kPrepChunk = 256;
@@ -73,6 +76,7 @@ struct ScParams {
     double e0;                // photopeak energy (Phys[7])
     double kn_total;          // KN Riemann sum over [0, pi] (reference normalisation)
     double h;                 // KN step (0.01f)
+    const double2* kn_tab;    // (cos n h, sin n h), n = 0 .. kKnSteps
     int n_det;
     int det_begin, det_count; // detector rows of this launch
     int max_sites;            // pair table stride per row

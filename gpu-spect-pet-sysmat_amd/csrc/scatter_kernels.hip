@@ -477,17 +477,30 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
         cmin = fmin(cmin, cq);
         cmax = fmax(cmax, cq);
     }
-    double th_min = acos(cmax);
-    const double th_max = acos(cmin);
+    // theta_min = acos(cmax), theta_max = acos(cmin); theta_min = 0 when the extended
+    // image -> site line passes through crystal r (scatter.cu:1298-1301, 1332-1335)
+    bool through = false;
     if (c >= (double)S.c_fwd || c <= -(double)S.c_fwd) {
-        // the extended image -> site line through crystal r (scatter.cu:1298-1301, 1332-1335)
         const double f = 1000.0 * ima;
         const double tx = S.ssx + f * asx, ty = S.ssy + f * asy, tz = S.ssz + f * asz;
-        const double through = chord_box64(q.xIs, q.yIs, q.zIs, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz);
-        if (through > (double)0.001f) th_min = 0.0;
+        through = chord_box64(q.xIs, q.yIs, q.zIs, tx, ty, tz, -R.hx, -R.hy, -R.hz, R.hx, R.hy, R.hz) >
+                  (double)0.001f;
     }
-    const int nstep = (int)((th_max - th_min) / p.h);
-    const double ratio = kn_sum(kc, th_min, nstep, p.h) / p.kn_total;
+    // cos / sin of theta_min directly, theta_max - theta_min from one atan2
+    const double smax = sqrt(fmax(0.0, 1.0 - cmin * cmin));  // sin theta_max
+    double ua, sa, dth;
+    if (through) {
+        ua = 1.0;
+        sa = 0.0;
+        dth = atan2(smax, cmin);
+    } else {
+        ua = cmax;
+        sa = sqrt(fmax(0.0, 1.0 - cmax * cmax));
+        dth = atan2(smax * cmax - cmin * sa, cmin * cmax + smax * sa);
+    }
+    const int nstep = min(kKnSteps, max(0, (int)(dth / p.h)));
+    const double2 cs = p.kn_tab[nstep];
+    const double ratio = kn_sum_cs(kc, ua, sa, cs.x, cs.y, p.h) / p.kn_total;
     // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
     const double x = e0 / es, x3 = x * x * x;
     const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
