@@ -160,7 +160,7 @@ def scatter_c3(dev, log):
                   "crystal_scatter_s": round(t_cry, 3), "collimator_scatter_s": round(t_col, 3)},
         "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
                       "apply_s": round(t_apply, 3), "fallback_tiles": n_fb,
-                      "fallback_frac": round(n_fb / (N * ((V + 63) // 64)), 4),
+                      "fallback_frac": round(n_fb / (N * sur.n_tiles), 4),
                       "train_pairs": tr["pairs"]},
         "collimator_cells": cells,
         "crystal_scatter_speedup_excl_training": round(t_cry / t_apply, 2),
@@ -173,6 +173,60 @@ def scatter_c3(dev, log):
     plan.close()
     sur.close()
     del pe, cry, col, out
+    torch.cuda.empty_cache()
+    return res
+
+
+def scatter_c4(dev, log):
+    """Batched variants: 6 collimator variants (seed 0) sharing a 3 x 48x48 detector and a
+    48^3 x 0.75 mm image; per variant PE (1, 2x2x2), then crystal + collimator scatter of
+    every variant from one captured hipGraph (and each variant in a graph of its own)."""
+    import torch
+    from sysmat import params
+    from sysmat.pe import PePlan
+    from sysmat.scatter import ScatterBatch
+    vs = params.config4_variants(seed=0)
+    N, V = vs[0].n_det, vs[0].n_vox
+    pes, pe_t = [], []
+    for g in vs:
+        pe = torch.empty((N, V), dtype=torch.float32, device=dev)
+        plan = PePlan(g, voxel_sub=1, sub=(2, 2, 2))
+        t0 = sync_t()
+        plan.launch(0, N, pe)
+        plan.sync()
+        pe_t.append(sync_t() - t0)
+        plan.close()
+        pes.append(pe)
+    log(f"C4 PE per variant {[round(t, 2) for t in pe_t]}")
+    outs = [torch.empty((N, V), dtype=torch.float32, device=dev) for _ in vs]
+    per = []
+    for i, g in enumerate(vs):
+        b = ScatterBatch([g])
+        t0 = sync_t()
+        b.run([pes[i]], [outs[i]], use_graph=True)
+        per.append(sync_t() - t0)
+        b.close()
+        log(f"C4 variant {i}: scatter {per[-1]:.2f}s")
+    b = ScatterBatch(vs)
+    t0 = sync_t()
+    b.run(pes, outs, use_graph=True)
+    t_all = sync_t() - t0
+    nodes = b.graph_nodes
+    log(f"C4 all variants in one graph ({nodes} nodes, capture + run): {t_all:.2f}s")
+    t0 = sync_t()
+    b.run(pes, outs, use_graph=True)  # replay of the captured graph
+    t_replay = sync_t() - t0
+    b.close()
+    log(f"C4 graph replay: {t_replay:.2f}s")
+    elems = len(vs) * N * V
+    res = {"workload": "6 collimator variants (1-3 plates, 4x4..12x12 holes), shared 3 x 48x48 detector (6912 det), "
+                       "48^3 x 0.75 mm image; PE (1, 2x2x2) + crystal + collimator scatter",
+           "elements": elems, "pe_s": [round(t, 3) for t in pe_t],
+           "scatter_s_per_variant_graph": [round(t, 3) for t in per],
+           "scatter_s_all_one_graph": round(t_all, 3), "scatter_s_replay": round(t_replay, 3),
+           "graph_nodes": nodes,
+           "pe_plus_scatter_elements_per_s": round(elems / (sum(pe_t) + t_replay), 1)}
+    del pes, outs
     torch.cuda.empty_cache()
     return res
 
@@ -239,7 +293,7 @@ def scatter_c5(dev, log, sample_rows=512, apply_block=2048):
                   "collimator_scatter_s": round(t_col, 3), "crystal_scatter_s_projected": round(t_cry_full, 2)},
         "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
                       "apply_s": round(t_ap, 3),
-                      "fallback_frac": round(n_fb / (N * ((V + 63) // 64)), 4),
+                      "fallback_frac": round(n_fb / (N * sur.n_tiles), 4),
                       "train_pairs": tr["pairs"]},
         "crystal_scatter_speedup_incl_training": round(t_cry_full / t_sur_full, 2),
         "pe_plus_scatter_speedup": round(plain_t / sur_t, 2),
@@ -265,6 +319,7 @@ def main():
     ap.add_argument("--block", type=int, default=1024, help="detector rows per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scatter", action="store_true", help="skip the PE+scatter legs")
+    ap.add_argument("--c4", action="store_true", help="also run the batched collimator-variant leg (minutes)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline rows (0: one per host core, <= 64)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3)")
@@ -381,6 +436,18 @@ def main():
         except Exception as e:
             pe_scatter["c5"] = {"error": repr(e)}
 
+
+    if args.c4:
+        if pe_scatter is None:
+            plan.close()
+            del out
+            torch.cuda.empty_cache()
+            pe_scatter = {}
+        try:
+            pe_scatter["c4"] = scatter_c4(dev, log)
+        except Exception as e:
+            pe_scatter["c4"] = {"error": repr(e)}
+
     cpu = None
     if not args.no_cpu:
         rng = np.random.default_rng(0)
@@ -415,7 +482,7 @@ def main():
     if cpu:
         rec["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(rec))
-    if args.no_scatter:
+    if args.no_scatter and not args.c4:
         plan.close()
 
 

@@ -335,7 +335,7 @@ static int run_blocks(const ScatterGeom& G, ScParams p, const unsigned* bits_dev
         p.det_begin = det_begin + b0;
         p.det_count = cnt;
         p.row_stride = row_stride;
-        p.n_tiles = (int)((p.V + 63) / 64);
+        set_tiles(p, true);
         p.n_rgroups = (cnt + 3) / 4;
         p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
         HIP_TRY(launch_scatter_prep(p, G.rows.ptr, G.cry.ptr, G.hol.ptr, bits_dev, pairs.ptr, counts.ptr, s));
@@ -359,7 +359,7 @@ static int run_scatter_list(ScatterGeom& G, int mode, const unsigned* bits, cons
     p.row_stride = row_stride;
     p.det_begin = det_begin;
     p.dst_by_row = dst_by_row ? 1 : 0;
-    p.n_tiles = (int)((p.V + 63) / 64);
+    set_tiles(p, true);  // listed tiles are 4 x 4 x 4 bricks too
     const long long pair_budget = 2048ll << 20;
     const int chunk = (int)std::max(1ll, std::min<long long>((long long)rows.size(),
                                                              pair_budget / ((long long)p.max_sites * sizeof(ScPair))));
@@ -1009,7 +1009,7 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     q.mu_y = 0.0f;
     q.sig_y = 1.0f;
     q.phi = 0.0f;
-    q.n_tiles = (b.V + 63) / 64;
+    q.n_tiles = (long long)((b.X + 3) / 4) * ((b.Y + 3) / 4) * ((b.Z + 3) / 4);
     q.ftab = S->ftab.ptr;
     *out = S;
     return SM_OK;
@@ -1057,7 +1057,7 @@ int sm_surrogate_near(SmSurrogate* S, const float* pe_dev, int det_begin, int de
     p.max_sites = S->q.n_keep;
     p.site_cap = S->n_near;
     p.row_stride = row_stride > 0 ? row_stride : p.V;
-    p.n_tiles = (int)((p.V + 63) / 64);
+    set_tiles(p, true);
     p.n_rgroups = (det_count + 3) / 4;
     p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
     HIP_TRY(launch_scatter(p, S->G.rows.ptr, S->keep.ptr + (long long)det_begin * S->q.n_keep,
@@ -1147,6 +1147,8 @@ long long sm_surrogate_fallback(SmSurrogate* S, const float* pe_dev, int det_beg
     return (long long)items.size();
 }
 
+long long sm_surrogate_tile_count(const SmSurrogate* S) { return S ? S->q.n_tiles : (long long)SM_E_ARG; }
+
 int sm_surrogate_destroy(SmSurrogate* S) {
     if (S) {
         use_device(S->G.device);
@@ -1200,7 +1202,7 @@ int batch_enqueue(SmScatterBatch& B, const float* const* pe, float* const* dst, 
                 p.det_begin = b0;
                 p.det_count = cnt;
                 p.row_stride = p.V;
-                p.n_tiles = (int)((p.V + 63) / 64);
+                set_tiles(p, true);
                 p.n_rgroups = (cnt + 3) / 4;
                 p.n_blocks = (long long)p.n_tiles * p.n_rgroups;
                 p.accumulate = term;

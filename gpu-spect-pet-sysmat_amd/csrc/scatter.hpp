@@ -86,7 +86,8 @@ struct ScParams {
                               // 3 collimator plate cells (source weights in the `pe` table, [cell][voxel])
     double w_col;             // collimator source factor Col[17] / Col[15]
     double fov2c;
-    int n_tiles;              // voxel tiles of 64
+    int n_tiles;              // voxel tiles: 4x4x4 bricks (brick = 1) or runs of 64 voxels
+    int brick;                // 1: wave = 4 x 4 x 4 voxel brick; 0: wave = 64 consecutive voxels
     int n_rgroups;            // detector groups of 4
     long long n_blocks;       // tiles * rgroups (before XCD padding)
     // optional modes (zero / null = off)
@@ -110,6 +111,12 @@ struct ScParams {
     const CellHole* choles;
     int n_cplates;
 };
+
+// Voxel tiles of the accumulation kernel.
+inline void set_tiles(ScParams& p, bool brick) {
+    p.brick = brick ? 1 : 0;
+    p.n_tiles = brick ? ((p.X + 3) / 4) * ((p.Y + 3) / 4) * ((p.Z + 3) / 4) : (int)((p.V + 63) / 64);
+}
 
 // Chunks of a row's pair table and their size.
 __host__ __device__ inline int site_chunks(const ScParams& p) {

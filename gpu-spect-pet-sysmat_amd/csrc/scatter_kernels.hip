@@ -535,14 +535,44 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
         rr = (int)(idx % p.n_rgroups) * 4 + wave;
         if (rr >= p.det_count) return;
     }
-    const int r = prep_row(p, rr);
+    // one detector row and one voxel tile per wave: tell the compiler, so the pair records
+    // are read through the scalar cache and the per-site skip below is a scalar branch
+    rr = __builtin_amdgcn_readfirstlane(rr);
+    tile = __builtin_amdgcn_readfirstlane(tile);
+    const int r = __builtin_amdgcn_readfirstlane(prep_row(p, rr));
     const long long drow = (p.row_list && p.dst_by_row) ? (long long)(r - p.det_begin) : (long long)rr;
-    const long long v = (long long)tile * 64 + lane;
-    const bool active = v < p.V;
+    long long v;
+    bool active;
+    if (p.brick) {  // 4 x 4 x 4 voxel brick, x fastest
+        const int nbx = (p.X + 3) >> 2, nby = (p.Y + 3) >> 2;
+        const int bx = tile % nbx, t2 = tile / nbx, by = t2 % nby, bz = t2 / nby;
+        const int ix = 4 * bx + (lane & 3), iy = 4 * by + ((lane >> 2) & 3), iz = 4 * bz + (lane >> 4);
+        active = ix < p.X && iy < p.Y && iz < p.Z;
+        v = ((long long)iz * p.Y + iy) * p.X + ix;
+    } else {        // 64 consecutive voxels
+        v = (long long)tile * 64 + lane;
+        active = v < p.V;
+    }
     const ScRow R = rows[r];
     double acc = 0.0;
-    if (R.use != 0.0f && active) {
-        const VoxelPt q = voxel_of(p, R, v);
+    if (R.use != 0.0f) {  // uniform: one detector row per wave
+        const VoxelPt q = voxel_of(p, R, active ? v : 0);
+        // bounding sphere of the wave's voxel centres (FP32, widened by 1e-3 mm)
+        float mnx = active ? (float)q.xI : INFINITY, mxx = active ? (float)q.xI : -INFINITY;
+        float mny = active ? (float)q.yI : INFINITY, mxy = active ? (float)q.yI : -INFINITY;
+        float mnz = active ? (float)q.zI : INFINITY, mxz = active ? (float)q.zI : -INFINITY;
+#pragma unroll
+        for (int o = 32; o >= 1; o >>= 1) {
+            mnx = fminf(mnx, __shfl_xor(mnx, o, 64)); mxx = fmaxf(mxx, __shfl_xor(mxx, o, 64));
+            mny = fminf(mny, __shfl_xor(mny, o, 64)); mxy = fmaxf(mxy, __shfl_xor(mxy, o, 64));
+            mnz = fminf(mnz, __shfl_xor(mnz, o, 64)); mxz = fmaxf(mxz, __shfl_xor(mxz, o, 64));
+        }
+        const float tcx = 0.5f * (mnx + mxx), tcy = 0.5f * (mny + mxy), tcz = 0.5f * (mnz + mxz);
+        const float trad = 0.5f * sqrtf((mxx - mnx) * (mxx - mnx) + (mxy - mny) * (mxy - mny) +
+                                        (mxz - mnz) * (mxz - mnz)) + 1e-3f;
+        // scatter angles outside [th_lo, th_hi] fail the energy-window test of the site term;
+        // 2 mrad covers the FP32 rounding of the bound below (at most ~5e-4 rad near cos = 1)
+        const float thl = (float)R.th_lo - 2e-3f, thh = (float)R.th_hi + 2e-3f;
         const KnCoef kc = kn_coef(p.e0);
         const int nch = site_chunks(p), csz = site_chunk_size(p);
         int cap = (p.site_cap > 0) ? p.site_cap : 0x7fffffff;
@@ -553,7 +583,21 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
             cap -= n_c;
             const ScPair* P = pairs + (long long)rr * p.max_sites + (long long)c * csz;
             for (int j = 0; j < n_c; ++j) {
-                const ScPair S = P[j];
+                const ScPair& S = P[j];
+                // skip the site for the whole wave when every voxel's scatter angle lies outside
+                // the window band: for I within trad of the centre T the angle between S - I and
+                // S - T is at most asin(trad / |S - T|)
+                const float ax = (float)S.sx - tcx, ay = (float)S.sy - tcy, az = (float)S.sz - tcz;
+                const float la2 = ax * ax + ay * ay + az * az;
+                if (la2 > trad * trad) {
+                    const float ila = __builtin_amdgcn_rsqf(la2);
+                    const float c0 =
+                        fminf(1.0f, fmaxf(-1.0f, (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * ila));
+                    const float th0 = acosf(c0);
+                    const float dth = asinf(fminf(1.0f, trad * ila * 1.0001f));
+                    if (th0 - dth > thh || th0 + dth < thl) continue;
+                }
+                if (!active) continue;
                 const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
                 acc += site_term(p, R, S, q, kc, w);
             }

@@ -67,8 +67,21 @@ __global__ void __launch_bounds__(256) surrogate_features_kernel(SgParams q, con
     for (int k = 0; k < kSgFeatures; ++k) x[i * kSgFeatures + k] = f[k];
 }
 
-// One wave = 32 consecutive voxels of one detector row.  Features go through
-// LDS into the MFMA B-operand layout; every member runs register-resident.
+// Voxel i (0..31) of wave chunk `ch` of a row: half (ch & 1) of the 4 x 4 x 4
+// brick ch / 2 (x fastest, the half holds z-layers 2 (ch & 1) and 2 (ch & 1) + 1).
+__device__ inline long long sg_voxel(const SgParams& q, long long ch, int i, bool& ok) {
+    const int nbx = (q.X + 3) >> 2, nby = (q.Y + 3) >> 2;
+    const long long b = ch >> 1;
+    const int bx = (int)(b % nbx), b2 = (int)(b / nbx), by = b2 % nby, bz = b2 / nby;
+    const int ix = 4 * bx + (i & 3), iy = 4 * by + ((i >> 2) & 3), iz = 4 * bz + 2 * (int)(ch & 1) + (i >> 4);
+    ok = ix < q.X && iy < q.Y && iz < q.Z;
+    return ((long long)iz * q.Y + iy) * q.X + ix;
+}
+
+// One wave = half a 4 x 4 x 4 voxel brick (32 voxels) of one detector row.
+// Features go through LDS into the MFMA B-operand layout; every member runs
+// register-resident.  Disagreeing members flag the brick (tile) for the exact
+// kernel, which prunes scatter sites per brick.
 __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
                                                               const ScPair* __restrict__ keep,
                                                               const int* __restrict__ keep_cnt,
@@ -77,18 +90,19 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
                                                               uint8_t* __restrict__ flags) {
     __shared__ float s_f[4][32][kSgFeatures + 1];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j16 = lane & 15;
-    const long long chunks = (q.V + 31) / 32;
+    const long long chunks = 2 * q.n_tiles;
     const long long gw = (long long)blockIdx.x * 4 + w;
     const int rr = (int)(gw / chunks);
-    const long long v0 = (gw % chunks) * 32;
+    const long long ch = gw % chunks;
     const bool valid_wave = rr < q.det_count;
     const int r = q.det_begin + (valid_wave ? rr : 0);
     const SgRow R = rows[r];
     const bool use = valid_wave && R.use != 0.0f;
     if (lane < 32) {
-        const long long v = v0 + lane;
+        bool ok;
+        const long long v = sg_voxel(q, ch, lane, ok);
         float f[kSgFeatures];
-        if (use && v < q.V) {
+        if (use && ok) {
             sg_features(q, R, keep + (long long)r * q.n_keep, keep_cnt[r], r, v, pe, dst[rr * q.row_stride + v], f);
         } else {
 #pragma unroll
@@ -122,8 +136,9 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
     if (g == 0 && use) {
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const long long v = v0 + 16 * t + j16;
-            if (v >= q.V) continue;
+            bool ok;
+            const long long v = sg_voxel(q, ch, 16 * t + j16, ok);
+            if (!ok) continue;
             float* o = dst + rr * q.row_stride + v;
             const float s_hat = *o + far_sum[t] / (float)q.n_members;
             if (q.n_members > 1 && s_hat > q.floor_abs && (far_max[t] - far_min[t]) > q.tol * s_hat) flag = true;
@@ -132,12 +147,13 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
     } else if (g == 0 && valid_wave) {
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            const long long v = v0 + 16 * t + j16;
-            if (v < q.V) dst[rr * q.row_stride + v] = 0.0f;
+            bool ok;
+            const long long v = sg_voxel(q, ch, 16 * t + j16, ok);
+            if (ok) dst[rr * q.row_stride + v] = 0.0f;
         }
     }
     const unsigned long long any = __ballot(flag);
-    if (lane == 0 && any && flags) flags[(long long)rr * q.n_tiles + v0 / 64] = 1;
+    if (lane == 0 && any && flags) flags[(long long)rr * q.n_tiles + (ch >> 1)] = 1;
 }
 
 hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
@@ -152,7 +168,7 @@ hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const
 
 hipError_t launch_surrogate_apply(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                   const uint8_t* packed, const float* pe, float* dst, uint8_t* flags, hipStream_t s) {
-    const long long per_row = (q.V + 31) / 32;  // waves per detector row
+    const long long per_row = 2 * q.n_tiles;  // waves per detector row: two per brick
     if (per_row <= 0 || q.det_count <= 0) return hipSuccess;
     // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
     const long long max_waves = ((1LL << 32) / 256 - 1) * 4;

@@ -7,7 +7,7 @@
 // S_far = S - S_near, is smooth in (r, v) and is predicted by the MLP from 12
 // features; the next n_ring sites give a fixed-weight prior feature
 // P_ring = sum_j PE[site_j, v] w_j.  Members of an ensemble that disagree
-// mark their 64-voxel tile for the exact kernel.
+// mark their 4 x 4 x 4 voxel brick for the exact kernel.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -36,7 +36,7 @@ struct SgParams {
     float pe_scale, s_scale, mu_y, sig_y, phi;  // normalisation (set after training-data statistics)
     float tol, floor_abs;
     int n_members;
-    long long n_tiles;          // 64-voxel tiles per row
+    long long n_tiles;          // 4 x 4 x 4 voxel bricks per row
     const float* ftab;          // [n_det][kSgTab + 1] angular factor tables
 };
 

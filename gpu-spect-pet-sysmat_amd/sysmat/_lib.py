@@ -78,6 +78,7 @@ SIGNATURES = {
     "sm_scatter_plan_cells": (ctypes.c_int, [_VP, ctypes.c_float, ctypes.c_float, ctypes.c_int, ctypes.c_int, _VP,
                                              ctypes.c_longlong, ctypes.c_int, _VP]),
     "sm_scatter_cell_table": (ctypes.c_longlong, [_VP, ctypes.c_float, ctypes.c_float, _VP, ctypes.c_longlong]),
+    "sm_surrogate_tile_count": (ctypes.c_longlong, [_VP]),
     "sm_csr_beams_device": (ctypes.c_int, [_VP, _VP, _VP, ctypes.c_int, ctypes.c_double, _VP, _VP, _VP]),
     "sm_scatter_cells": (ctypes.c_int, [_VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                         ctypes.c_int, _VP, ctypes.c_int]),

@@ -13,7 +13,7 @@ remainder is smooth in (r, v).  This module
    field S - S_near of a seeded sample of detector rows computed by the exact
    kernel;
 3. predicts the far field for every element (fused feature + MLP kernel),
-   recomputes 64-voxel tiles where members disagree with the exact kernel,
+   recomputes 4x4x4-voxel bricks where members disagree with the exact kernel,
    and judges the result on a disjoint seeded set of held-out rows.
 
 Only device tensors (PyTorch-ROCm) and C ABI calls are used; there is no CPU
@@ -68,7 +68,7 @@ class TrainConfig:
     lr: float = 2e-3
     seed: int = 0
     phi_rel: float = 1e-4          # log floor relative to the largest training element
-    tol: float = 0.1               # member disagreement that sends a tile to the exact kernel
+    tol: float = 0.02              # member disagreement (fraction of the element) that sends a tile to the exact kernel
     floor_rel: float = 1e-3        # disagreement is only checked above this fraction of the scale
 
 
@@ -89,6 +89,7 @@ class ScatterSurrogate:
         check(self.L.sm_surrogate_create(ptr(self._keep[0]), ptr(self._keep[1]), ptr(self._keep[2]), rotation, None,
                                          1 if use_bitmap else 0, n_near, n_ring, device, ctypes.byref(h)))
         self._h = h
+        self.n_tiles = int(check(self.L.sm_surrogate_tile_count(h)))  # 4x4x4 voxel bricks per row
         self.members: List[mlp.Mlp] = []
         self.packed = None
         self.norm = None
@@ -231,7 +232,7 @@ class ScatterSurrogate:
             raise RuntimeError("surrogate not trained")
         nm = len(self.members) if n_members is None else n_members
         self.near(pe, det_begin, det_count, dst, row_stride, stream)
-        n_tiles = (self.geo.n_vox + 63) // 64
+        n_tiles = self.n_tiles
         flags = torch.zeros(det_count * n_tiles, dtype=torch.uint8, device=pe.device) if fallback else None
         floor_abs = self.cfg.floor_rel * self.norm["s_scale"]
         tol = self.cfg.tol if tol is None else tol
@@ -273,7 +274,7 @@ class ScatterSurrogate:
             "near_only_total_rel_err": float((near.double().sum().item() - tot) / tot) if tot > 0 else 0.0,
             "near_only_l1_rel_err": float((near.double() - e).abs().sum().item() / tot) if tot > 0 else 0.0,
             "fallback_tiles": n_fb,
-            "tiles": len(rows) * ((V + 63) // 64),
+            "tiles": len(rows) * self.n_tiles,
         }
         return out
 

@@ -262,8 +262,8 @@ int sm_scatter_batch_destroy(SmScatterBatch* b);
  * the largest single-site weight at the image centre are ranked once; the
  * n_near best are summed exactly for every voxel (S_near), the MLP predicts the
  * rest from 12 features (one uses the fixed-weight ring of the next n_ring
- * sites), ensemble members that disagree send their 64-voxel tile to the exact
- * kernel.  pe_dev: the full N x V PE matrix of the rotation (row stride V). */
+ * sites), ensemble members that disagree send their 4 x 4 x 4 voxel brick to the
+ * exact kernel.  pe_dev: the full N x V PE matrix of the rotation (row stride V). */
 typedef struct SmSurrogate SmSurrogate;
 int sm_surrogate_create(const float* det, const float* img, const float* phy, int rotation,
                         const unsigned* crystal_words, int use_bitmap, int n_near, int n_ring, int device,
@@ -283,9 +283,12 @@ int sm_surrogate_exact_rows(SmSurrogate* s, const float* pe_dev, const int* rows
  * (NULL: 0..n_vox-1), S_near of those rows in near_dev; x_dev[(row*n_vox + j)*12 + f]. */
 int sm_surrogate_features(SmSurrogate* s, const float* pe_dev, const float* near_dev, long long near_stride,
                           int det_begin, int det_count, const int* vox_dev, int n_vox, float* x_dev, void* stream);
+/* Voxel tiles per row: 4 x 4 x 4 bricks, ceil(X/4) ceil(Y/4) ceil(Z/4). */
+long long sm_surrogate_tile_count(const SmSurrogate* s);
 /* dst_dev holds S_near on entry and S_near + predicted far field on exit;
  * tiles where members differ by more than tol x value (value > floor_abs) are
- * marked in tile_flags_dev (det_count x ceil(V/64) bytes, zeroed by the caller). */
+ * marked in tile_flags_dev (det_count x sm_surrogate_tile_count bytes, zeroed by
+ * the caller). */
 int sm_surrogate_apply(SmSurrogate* s, const void* packed_dev, int n_members, const float* pe_dev, int det_begin,
                        int det_count, float* dst_dev, long long row_stride, float tol, float floor_abs,
                        unsigned char* tile_flags_dev, void* stream);
