@@ -27,6 +27,8 @@ struct PePlanImpl {
     DeviceBuf<Plate> pla;
     DeviceBuf<Slab> slb;
     DeviceBuf<int> status;
+    DeviceBuf<long long> cry_ptr, hol_ptr;      // per-detector candidate lists (launch_detector_cull)
+    DeviceBuf<unsigned short> cry_list, hol_list;
     PeParams base{};
     hipStream_t last_stream = nullptr;
     int tile_fixed = 0;
@@ -37,6 +39,7 @@ struct PePlanImpl {
         hipStream_t stream;
     };
     std::vector<Launch> pending;  // launches since the last sync (re-run with smaller tiles on overflow)
+    PeLists lists() const { return PeLists{cry_ptr.ptr, cry_list.ptr, hol_ptr.ptr, hol_list.ptr}; }
 };
 
 static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, PeParams& p) {
@@ -82,13 +85,22 @@ static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, 
     return SM_OK;
 }
 
-static void set_tiles(PeParams& p, int tx, int ty, int tz) {
+// Voxel tiles of tx x ty x tz, grouped g x g x g per workgroup when the whole
+// plan (every detector of the table x every tile) is large enough.
+static void set_tiles(PeParams& p, int tx, int ty, int tz, int g = kGroupTiles) {
     p.tx = std::max(1, std::min(tx, p.X));
     p.ty = std::max(1, std::min(ty, p.Y));
     p.tz = std::max(1, std::min(tz, p.Zslab));
     p.ntx = (p.X + p.tx - 1) / p.tx;
     p.nty = (p.Y + p.ty - 1) / p.ty;
     p.ntz = (p.Zslab + p.tz - 1) / p.tz;
+    if ((long long)p.n_det * p.ntx * p.nty * p.ntz < kGroupMinTiles) g = 1;
+    p.gx = std::max(1, std::min(g, p.ntx));
+    p.gy = std::max(1, std::min(g, p.nty));
+    p.gz = std::max(1, std::min(g, p.ntz));
+    p.ngx = (p.ntx + p.gx - 1) / p.gx;
+    p.ngy = (p.nty + p.gy - 1) / p.gy;
+    p.ngz = (p.ntz + p.gz - 1) / p.gz;
 }
 
 static int upload(PePlanImpl& P, const float* col, const float* det, const float* img) {
@@ -103,20 +115,53 @@ static int upload(PePlanImpl& P, const float* col, const float* det, const float
     return SM_OK;
 }
 
-// Launch with the plan's tile; on cull-list overflow halve the tile and
-// relaunch (synchronous).  Returns SM_OK or an error.
+// Per-detector candidate lists for the plan's image slab and rotation: count
+// pass, host prefix sums, fill pass (synchronous, once per plan).
+static int build_lists(PePlanImpl& P) {
+    const PeParams& p = P.base;
+    const int n = p.n_det;
+    DeviceBuf<int> counts;
+    int rc;
+    if ((rc = counts.alloc(2 * (size_t)n)) < 0) return rc;
+    if ((rc = P.cry_ptr.alloc((size_t)n + 1)) < 0) return rc;
+    if ((rc = P.hol_ptr.alloc((size_t)n + 1)) < 0) return rc;
+    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, 0, counts.ptr, P.lists(), nullptr));
+    std::vector<int> c(2 * (size_t)n);
+    HIP_TRY(hipMemcpy(c.data(), counts.ptr, c.size() * sizeof(int), hipMemcpyDeviceToHost));
+    std::vector<long long> cp((size_t)n + 1, 0), hp((size_t)n + 1, 0);
+    for (int d = 0; d < n; ++d) {
+        cp[d + 1] = cp[d] + c[2 * (size_t)d];
+        hp[d + 1] = hp[d] + c[2 * (size_t)d + 1];
+    }
+    HIP_TRY(hipMemcpy(P.cry_ptr.ptr, cp.data(), cp.size() * sizeof(long long), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(P.hol_ptr.ptr, hp.data(), hp.size() * sizeof(long long), hipMemcpyHostToDevice));
+    if ((rc = P.cry_list.alloc((size_t)cp[n])) < 0) return rc;
+    if ((rc = P.hol_list.alloc((size_t)hp[n])) < 0) return rc;
+    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, 1, counts.ptr, P.lists(), nullptr));
+    HIP_TRY(hipDeviceSynchronize());
+    return SM_OK;
+}
+
+// Launch with the plan's tile; on cull-list overflow halve the tile (at a
+// single voxel: drop the tile grouping) and relaunch (synchronous).  Returns
+// SM_OK or an error.
 static int run_sync(PePlanImpl& P, PeParams p, float* dst, hipStream_t stream) {
     int t[3] = {p.tx, p.ty, p.tz};
+    int g = kGroupTiles;
     for (;;) {
-        set_tiles(p, t[0], t[1], t[2]);
+        set_tiles(p, t[0], t[1], t[2], g);
         HIP_TRY(hipMemsetAsync(P.status.ptr, 0, sizeof(int), stream));
-        HIP_TRY(launch_pe(p, P.cry.ptr, P.hol.ptr, P.pla.ptr, P.slb.ptr, dst, P.status.ptr, stream));
+        HIP_TRY(launch_pe(p, P.cry.ptr, P.hol.ptr, P.pla.ptr, P.slb.ptr, P.lists(), dst, P.status.ptr, stream));
         int st = 0;
         HIP_TRY(hipMemcpyAsync(&st, P.status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
         HIP_TRY(hipStreamSynchronize(stream));
         if (!(st & 1)) return SM_OK;
-        if (t[0] == 1 && t[1] == 1 && t[2] == 1)
-            return set_error(SM_E_OVERFLOW, "cull list exceeds 4096 crystals / 1024 holes for a single voxel");
+        if (t[0] == 1 && t[1] == 1 && t[2] == 1) {
+            if (g == 1)
+                return set_error(SM_E_OVERFLOW, "cull list exceeds 4096 crystals / 1024 holes for a single voxel");
+            g = 1;
+            continue;
+        }
         for (int& v : t) v = std::max(1, v / 2);
     }
 }
@@ -159,6 +204,7 @@ int sm_pe_plan_create(const float* col, const float* det, const float* img, cons
     P->device = o->device;
     rc = upload(*P, col, det, img);
     if (rc >= 0) rc = fill_params(img, *o, P->T, P->base);
+    if (rc >= 0) rc = build_lists(*P);
     if (rc < 0) {
         delete P;
         return rc;
@@ -191,7 +237,7 @@ int sm_pe_plan_launch(SmPePlan* plan, int det_begin, int det_count, float* dst_d
     p.row_stride = row_stride > 0 ? row_stride : (long long)p.X * p.Y * p.Zslab;
     hipStream_t s = (hipStream_t)stream;
     P->last_stream = s;
-    HIP_TRY(launch_pe(p, P->cry.ptr, P->hol.ptr, P->pla.ptr, P->slb.ptr, dst_dev, P->status.ptr, s));
+    HIP_TRY(launch_pe(p, P->cry.ptr, P->hol.ptr, P->pla.ptr, P->slb.ptr, P->lists(), dst_dev, P->status.ptr, s));
     P->pending.push_back({det_begin, det_count, dst_dev, p.row_stride, s});
     return SM_OK;
 }

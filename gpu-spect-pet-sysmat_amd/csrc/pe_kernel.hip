@@ -8,9 +8,13 @@
 // per-element angular-sphere cull (PESysMatGen.cu:504-590).
 //
 // Schedule (MI355X-first, not the reference's one-thread-per-element grid):
-//   * one workgroup = 4 wave64s = one detector x one voxel tile;
-//   * the workgroup culls all crystals / holes once against the tile's
-//     bounding sphere (a proven superset of every element's cull list) into LDS;
+//   * once per plan, each detector's candidate crystals / holes for the whole
+//     image slab (pe_detector_cull_kernel);
+//   * one workgroup = 4 wave64s = one detector x a group of 2 x 2 x 2 voxel
+//     tiles (one tile when a launch has few tiles); the workgroup culls the
+//     detector's list against the group's voxel-centre sphere into LDS, then
+//     per tile that list against the tile's sphere (each a proven superset of
+//     every element's cull list);
 //   * one wave = one element at a time, its 64 lanes = 64 sub-rays.  Per
 //     element the wave filters the tile candidates lane-parallel -- first the
 //     "bundle" test (every sub-ray of the element bounded laterally in each
@@ -34,15 +38,27 @@ constexpr int kThreads = 64 * kWaves;
 constexpr int kListCap = 192;   // per-wave compacted attenuator boxes (beyond: bitmask path)
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
 
-__device__ inline float wave_min32(float v) {
-#pragma unroll
-    for (int o = 16; o >= 1; o >>= 1) v = fminf(v, __shfl_xor(v, o, 64));
-    return v;
+// Lane exchange within each row of 16 lanes (DPP): quad_perm [1,0,3,2],
+// quad_perm [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with
+// min / max, every lane of a row ends with the row's minimum / maximum.
+template <int kCtrl>
+__device__ inline float dpp(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
 }
-__device__ inline float wave_max32(float v) {
-#pragma unroll
-    for (int o = 16; o >= 1; o >>= 1) v = fmaxf(v, __shfl_xor(v, o, 64));
-    return v;
+__device__ inline float row_min16(float v) {
+    v = fminf(v, dpp<0xB1>(v));
+    v = fminf(v, dpp<0x4E>(v));
+    v = fminf(v, dpp<0x141>(v));
+    return fminf(v, dpp<0x140>(v));
+}
+__device__ inline float row_max16(float v) {
+    v = fmaxf(v, dpp<0xB1>(v));
+    v = fmaxf(v, dpp<0x4E>(v));
+    v = fmaxf(v, dpp<0x141>(v));
+    return fmaxf(v, dpp<0x140>(v));
+}
+__device__ inline float lane_value(float v, int l) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
 }
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
@@ -124,11 +140,146 @@ __device__ inline float crystal_chord(const WRay& R, const Crystal* __restrict__
     return chord_world(Rk, -K.hx, K.hx, -K.hy, K.hy, -K.hz, K.hz);
 }
 
-__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3))) pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry,
-                                                           const Hole* __restrict__ hol,
-                                                           const Plate* __restrict__ pla,
-                                                           const Slab* __restrict__ slb, float* __restrict__ dst,
-                                                           int* __restrict__ status) {
+// Superset cull of a sphere of voxel centres (centre t, radius rho) against
+// detector D.  For any voxel centre I within rho of t, the reference predicate
+// |u_k - u_d| <= r_k/(2 D_k) + r_d/(2 D_d) implies
+//   |u_k(t) - u_d(t)| <= r_k/(2 (D_k0 - rho)) + r_d/(2 (D_d0 - rho))
+//                        + 2 rho/(2 D_k0 - rho) + 2 rho/(2 D_d0 - rho)
+// by the Dunkl-Williams inequality |a/|a| - b/|b|| <= 2 |a - b| / (|a| + |b|)
+// with |a - b| <= rho, |a| >= D0 - rho, |b| = D0.
+// y-range: sub-points of crystal d lie at y <= cy + (t/2)(1 - 1/n_y); voxel
+// sub-points at y >= (voxel centre y) - pey.  When every sub-ray runs toward +y
+// by at least 0.01 mm (so no sub-ray takes the y-parallel branches), a crystal
+// whose y-range lies beyond the sub-points, or before the voxel sub-points, by
+// more than the 1e-3 mm margin (>> FP32 rounding of the slab parameters at these
+// distances) has a zero chord for every sub-ray: its y-slab interval starts
+// after |PQ| or ends before 0 (length_box_ray keeps only chords strictly inside
+// the segment, PESysMatGen.cu:19-151).
+struct SphereCull {
+    float tcx, tcy, tcz, rho;
+    float udx, udy, udz, crit_d;
+    float q_ymax, pymin;
+    bool yclip;
+};
+
+__device__ inline SphereCull sphere_cull(const PeParams& p, const Crystal& D, float tcx, float tcy, float tcz,
+                                         float rho) {
+    SphereCull c;
+    c.tcx = tcx; c.tcy = tcy; c.tcz = tcz; c.rho = rho;
+    const float hvx0 = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
+    const float hvy0 = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
+    const float pey0 = fabsf(p.rs) * hvx0 + fabsf(p.rc) * hvy0;
+    c.q_ymax = D.cy + D.hy * (1.0f - 1.0f / (float)p.cy) + 1e-3f;
+    c.pymin = tcy - rho - pey0 - 1e-3f;
+    c.yclip = (D.cy - D.hy) - (tcy + rho + pey0) > 0.01f;
+    const float ddx = D.cx - tcx, ddy = D.cy - tcy, ddz = D.cz - tcz;
+    const float Dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
+    c.udx = ddx / Dd; c.udy = ddy / Dd; c.udz = ddz / Dd;
+    c.crit_d = (Dd > rho) ? 0.5f * D.rbound / (Dd - rho) + 2.0f * rho / (2.0f * Dd - rho) : 4.0f;
+    return c;
+}
+
+__device__ inline bool sphere_take_crystal(const SphereCull& c, const Crystal* __restrict__ cry, int k) {
+    const float4 c0 = reinterpret_cast<const float4*>(cry + k)[0];
+    const float rb = cry[k].rbound;
+    const float khy = cry[k].hy;
+    const float kx = c0.x - c.tcx, ky = c0.y - c.tcy, kz = c0.z - c.tcz;
+    const float Dk = sqrtf(kx * kx + ky * ky + kz * kz);
+    if (c.yclip && (c0.y - khy >= c.q_ymax || c0.y + khy <= c.pymin)) return false;
+    if (Dk <= c.rho) return true;
+    const float iD = 1.0f / Dk;
+    const float bound = 0.5f * rb / (Dk - c.rho) + 2.0f * c.rho / (2.0f * Dk - c.rho) + c.crit_d;
+    const float vx = kx * iD - c.udx, vy = ky * iD - c.udy, vz = kz * iD - c.udz;
+    return sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
+}
+
+__device__ inline bool sphere_take_hole(const SphereCull& c, const Hole& H) {
+    const float hx = H.x - c.tcx, hy = 0.5f * (H.y1 + H.y2) - c.tcy, hz = H.z - c.tcz;
+    const float Dh = sqrtf(hx * hx + hy * hy + hz * hz);
+    if (Dh <= c.rho) return true;
+    const float iD = 1.0f / Dh;
+    const float bound = H.rbound / (Dh - c.rho) + 2.0f * c.rho / (2.0f * Dh - c.rho) + c.crit_d;
+    const float vx = hx * iD - c.udx, vy = hy * iD - c.udy, vz = hz * iD - c.udz;
+    return sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
+}
+
+// Bounding sphere of the voxel centres of the box [v0, v1) (slab-local z),
+// in the rotated image frame.
+__device__ inline void box_sphere(const PeParams& p, int vx0, int vy0, int vz0, int vx1, int vy1, int vz1, float& tcx,
+                                  float& tcy, float& tcz, float& rho) {
+    float ax, ay, az, bx, by, bz;
+    voxel_centre(p, vx0, vy0, p.z0 + vz0, ax, ay, az);
+    voxel_centre(p, vx1 - 1, vy1 - 1, p.z0 + vz1 - 1, bx, by, bz);
+    rotate_image(p, 0.5f * (ax + bx), 0.5f * (ay + by), tcx, tcy);
+    tcz = 0.5f * (az + bz);
+    rho = 0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
+}
+
+// Ordered block-wide compaction step: every thread offers `take`; returns the
+// slot of a taken item (from `base_count`) and adds the block's count to it.
+// Needs s_wcount[kWaves] and all threads of the block.
+__device__ inline int block_compact(bool take, int& base_count, int* s_wcount, int lane, int wave,
+                                    unsigned long long below) {
+    const unsigned long long bal = __ballot(take);
+    if (lane == 0) s_wcount[wave] = __popcll(bal);
+    __syncthreads();
+    int off = base_count, all = 0;
+    for (int w2 = 0; w2 < kWaves; ++w2) {
+        const int c = s_wcount[w2];
+        if (w2 < wave) off += c;
+        all += c;
+    }
+    base_count += all;
+    __syncthreads();
+    return off + __popcll(bal & below);
+}
+
+// Per-detector candidate lists for the whole image slab (one workgroup per
+// detector): the crystals and holes that pass the sphere cull of the slab's
+// voxel-centre sphere, in index order.  Every tile sphere lies inside the slab
+// sphere, and both are supersets of every element's exact cull list, so the
+// tile cull may scan this list instead of the whole table and still keep every
+// element's list (and its index order) unchanged.  fill = 0 counts into
+// counts[2 d], counts[2 d + 1]; fill = 1 writes the lists at the offsets.
+__global__ void __launch_bounds__(kThreads) pe_detector_cull_kernel(PeParams p, const Crystal* __restrict__ cry,
+                                                                    const Hole* __restrict__ hol, int fill,
+                                                                    int* __restrict__ counts, PeLists L) {
+    __shared__ int s_wcount[kWaves];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int d = blockIdx.x;
+    const Crystal D = cry[d];
+    int nc = 0, nh = 0;
+    if (D.mu_t > 0.01f) {
+        float tcx, tcy, tcz, rho;
+        box_sphere(p, 0, 0, 0, p.X, p.Y, p.Zslab, tcx, tcy, tcz, rho);
+        const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+        unsigned short* cl = fill ? L.cry_list + L.cry_ptr[d] : nullptr;
+        unsigned short* hl = fill ? L.hol_list + L.hol_ptr[d] : nullptr;
+        for (int base = 0; base < p.n_det; base += kThreads) {
+            const int k = base + tid;
+            const bool take = k < p.n_det && k != d && sphere_take_crystal(C, cry, k);
+            const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
+            if (take && fill) cl[slot] = (unsigned short)k;
+        }
+        for (int base = 0; base < p.n_holes; base += kThreads) {
+            const int h = base + tid;
+            const bool take = h < p.n_holes && sphere_take_hole(C, hol[h]);
+            const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
+            if (take && fill) hl[slot] = (unsigned short)h;
+        }
+    }
+    if (!fill && tid == 0) {
+        counts[2 * d] = nc;
+        counts[2 * d + 1] = nh;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3)))
+pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restrict__ hol, const Plate* __restrict__ pla,
+               const Slab* __restrict__ slb, PeLists L, float* __restrict__ dst, int* __restrict__ status) {
+    __shared__ unsigned short s_grp[kCapGroup];
+    __shared__ unsigned short s_hgrp[kCapHoles];
     __shared__ unsigned short s_cand[kCapCrystals];
     __shared__ unsigned short s_hcand[kCapHoles];
     __shared__ unsigned s_mask[kWaves][kCapCrystals / 32];
@@ -137,6 +288,7 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
     __shared__ int s_hol[kWaves][kHoleCap];
     __shared__ float4 s_box[kWaves][kMaxSlabs];
     __shared__ int s_wcount[kWaves];
+    __shared__ int s_meta[8];  // tile-group state, re-read per tile (keeps it out of scalar registers)
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -145,139 +297,80 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
 
     const long long wg = blockIdx.x;
     const int drow = (int)(wg % p.det_count);
-    const long long tile = wg / p.det_count;
+    const long long grp = wg / p.det_count;
     const int d = p.det_begin + drow;
-    const int tix = (int)(tile % p.ntx);
-    const int tiy = (int)((tile / p.ntx) % p.nty);
-    const int tiz = (int)(tile / ((long long)p.ntx * p.nty));
-    const int vx0 = tix * p.tx, vy0 = tiy * p.ty, vz0 = tiz * p.tz;
-    const int vx1 = min(vx0 + p.tx, p.X), vy1 = min(vy0 + p.ty, p.Y), vz1 = min(vz0 + p.tz, p.Zslab);
-    const int nvx = vx1 - vx0, nvy = vy1 - vy0;
-    const int nvox = nvx * nvy * (vz1 - vz0);
+    // tile group [tx0, tx1) x [ty0, ty1) x [tz0, tz1) of this workgroup
+    const int tx0 = (int)(grp % p.ngx) * p.gx;
+    const int ty0 = (int)((grp / p.ngx) % p.ngy) * p.gy;
+    const int tz0 = (int)(grp / ((long long)p.ngx * p.ngy)) * p.gz;
+    const int tx1 = min(tx0 + p.gx, p.ntx), ty1 = min(ty0 + p.gy, p.nty), tz1 = min(tz0 + p.gz, p.ntz);
+    const int gvx0 = tx0 * p.tx, gvy0 = ty0 * p.ty, gvz0 = tz0 * p.tz;
+    const int gvx1 = min(tx1 * p.tx, p.X), gvy1 = min(ty1 * p.ty, p.Y), gvz1 = min(tz1 * p.tz, p.Zslab);
 
     const Crystal D = cry[d];
     float* row = dst + (long long)drow * p.row_stride;
 
     if (!(D.mu_t > 0.01f)) {  // no culling, no sub-rays: the element is 0 (PESysMatGen.cu:504, 598)
+        const int nvx = gvx1 - gvx0, nvy = gvy1 - gvy0, nvox = nvx * nvy * (gvz1 - gvz0);
         for (int e = tid; e < nvox; e += kThreads) {
             const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
-            row[(long long)(vz0 + lz) * p.X * p.Y + (vy0 + ly) * p.X + (vx0 + lx)] = 0.0f;
+            row[(long long)(gvz0 + lz) * p.X * p.Y + (gvy0 + ly) * p.X + (gvx0 + lx)] = 0.0f;
         }
         return;
     }
 
-    // ---- tile bounding sphere of the voxel centres -----------------------
-    float ax, ay, az, bx, by, bz;
-    voxel_centre(p, vx0, vy0, p.z0 + vz0, ax, ay, az);
-    voxel_centre(p, vx1 - 1, vy1 - 1, p.z0 + vz1 - 1, bx, by, bz);
-    float tcx, tcy;
-    rotate_image(p, 0.5f * (ax + bx), 0.5f * (ay + by), tcx, tcy);
-    const float tcz = 0.5f * (az + bz);
-    const float rho =
-        0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
-
-    // ---- y-range of every sub-ray of the tile ------------------------------
-    // Sub-points of crystal d lie at y <= cy + (t/2)(1 - 1/n_y); voxel sub-points
-    // at y >= (voxel centre y) - pey.  When every sub-ray runs toward +y by at
-    // least 0.01 mm (so no sub-ray takes the y-parallel branches), a
-    // crystal whose y-range lies beyond the sub-points, or before the voxel
-    // sub-points, by more than the 1e-3 mm margin (>> FP32 rounding of the
-    // slab parameters at these distances) has a zero chord for every sub-ray:
-    // its y-slab interval starts after |PQ| or ends before 0
-    // (length_box_ray keeps only chords strictly inside the segment,
-    // PESysMatGen.cu:19-151).  Such crystals are dropped from the tile list.
-    const float hvx0 = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
-    const float hvy0 = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
-    const float pey0 = fabsf(p.rs) * hvx0 + fabsf(p.rc) * hvy0;
-    const float q_ymax = D.cy + D.hy * (1.0f - 1.0f / (float)p.cy) + 1e-3f;
-    const float tile_pymin = tcy - rho - pey0 - 1e-3f;
-    const bool tile_yclip = (D.cy - D.hy) - (tcy + rho + pey0) > 0.01f;
-
-    // ---- workgroup cull: superset of every element's list ----------------
-    // For any voxel centre I within rho of the tile centre I0, the reference
-    // predicate |u_k - u_d| <= r_k/(2 D_k) + r_d/(2 D_d) implies
-    // |u_k(I0) - u_d(I0)| <= r_k/(2 (D_k0 - rho)) + r_d/(2 (D_d0 - rho))
-    //                        + 2 rho/(2 D_k0 - rho) + 2 rho/(2 D_d0 - rho)
-    // by the Dunkl-Williams inequality |a/|a| - b/|b|| <= 2 |a - b| / (|a| + |b|)
-    // with |a - b| <= rho, |a| >= D0 - rho, |b| = D0.  Ordered compaction keeps
-    // the lists sorted by index, so every element visits attenuators in index order.
-    int nc = 0, nh = 0;
+    // ---- group cull: the group's voxel-centre sphere over the detector's
+    //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
+    //      then culls these shorter lists (a group of one tile writes its tile
+    //      lists directly).  Ordered compaction keeps every list sorted by index.
+    const bool grouped = (tx1 - tx0) * (ty1 - ty0) * (tz1 - tz0) > 1;
     {
-        const float ddx = D.cx - tcx, ddy = D.cy - tcy, ddz = D.cz - tcz;
-        const float Dd = sqrtf(ddx * ddx + ddy * ddy + ddz * ddz);
-        const float udx = ddx / Dd, udy = ddy / Dd, udz = ddz / Dd;
-        const float crit_d = (Dd > rho) ? 0.5f * D.rbound / (Dd - rho) + 2.0f * rho / (2.0f * Dd - rho) : 4.0f;
-        for (int base = 0; base < p.n_det; base += kThreads) {
-            const int k = base + tid;
+        int ng = 0, nhg = 0;
+        float tcx, tcy, tcz, rho;
+        box_sphere(p, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, tcx, tcy, tcz, rho);
+        const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+        const long long c0 = L.cry_ptr[d], c1 = L.cry_ptr[d + 1];
+        for (long long base = c0; base < c1; base += kThreads) {
+            const long long i = base + tid;
+            int k = 0;
             bool take = false;
-            if (k < p.n_det && k != d) {
-                const float4 c0 = reinterpret_cast<const float4*>(cry + k)[0];
-                const float rb = cry[k].rbound;
-                const float khy = cry[k].hy;
-                const float kx = c0.x - tcx, ky = c0.y - tcy, kz = c0.z - tcz;
-                const float Dk = sqrtf(kx * kx + ky * ky + kz * kz);
-                if (tile_yclip && (c0.y - khy >= q_ymax || c0.y + khy <= tile_pymin)) {
-                    take = false;
-                } else if (Dk <= rho) {
-                    take = true;
-                } else {
-                    const float iD = 1.0f / Dk;
-                    const float bound = 0.5f * rb / (Dk - rho) + 2.0f * rho / (2.0f * Dk - rho) + crit_d;
-                    const float vx = kx * iD - udx, vy = ky * iD - udy, vz = kz * iD - udz;
-                    take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
-                }
+            if (i < c1) {
+                k = (int)L.cry_list[i];
+                take = sphere_take_crystal(C, cry, k);
             }
-            const unsigned long long bal = __ballot(take);
-            if (lane == 0) s_wcount[wave] = __popcll(bal);
-            __syncthreads();
-            int off = nc, all = 0;
-            for (int w2 = 0; w2 < kWaves; ++w2) {
-                const int c = s_wcount[w2];
-                if (w2 < wave) off += c;
-                all += c;
+            const int slot = block_compact(take, ng, s_wcount, lane, wave, below);
+            if (take && slot < kCapGroup) {
+                if (grouped) s_grp[slot] = (unsigned short)k;
+                else s_cand[slot] = (unsigned short)k;
             }
-            const int slot = off + __popcll(bal & below);
-            if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
-            nc += all;
-            __syncthreads();
         }
-        for (int base = 0; base < p.n_holes; base += kThreads) {
-            const int h = base + tid;
+        const long long h0 = L.hol_ptr[d], h1 = L.hol_ptr[d + 1];
+        for (long long base = h0; base < h1; base += kThreads) {
+            const long long i = base + tid;
+            int h = 0;
             bool take = false;
-            if (h < p.n_holes) {
-                const Hole H = hol[h];
-                const float hx = H.x - tcx, hy = 0.5f * (H.y1 + H.y2) - tcy, hz = H.z - tcz;
-                const float Dh = sqrtf(hx * hx + hy * hy + hz * hz);
-                if (Dh <= rho) {
-                    take = true;
-                } else {
-                    const float iD = 1.0f / Dh;
-                    const float bound = H.rbound / (Dh - rho) + 2.0f * rho / (2.0f * Dh - rho) + crit_d;
-                    const float vx = hx * iD - udx, vy = hy * iD - udy, vz = hz * iD - udz;
-                    take = sqrtf(vx * vx + vy * vy + vz * vz) <= bound * 1.0001f + 1e-5f;
-                }
+            if (i < h1) {
+                h = (int)L.hol_list[i];
+                take = sphere_take_hole(C, hol[h]);
             }
-            const unsigned long long bal = __ballot(take);
-            if (lane == 0) s_wcount[wave] = __popcll(bal);
-            __syncthreads();
-            int off = nh, all = 0;
-            for (int w2 = 0; w2 < kWaves; ++w2) {
-                const int c = s_wcount[w2];
-                if (w2 < wave) off += c;
-                all += c;
+            const int slot = block_compact(take, nhg, s_wcount, lane, wave, below);
+            if (take && slot < kCapHoles) {
+                if (grouped) s_hgrp[slot] = (unsigned short)h;
+                else s_hcand[slot] = (unsigned short)h;
             }
-            const int slot = off + __popcll(bal & below);
-            if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
-            nh += all;
-            __syncthreads();
         }
-    }
-    if (nc > kCapCrystals || nh > kCapHoles) {  // host retries with smaller tiles
-        if (tid == 0) atomicOr(status, 1);
-        return;
+        if (ng > kCapGroup || nhg > kCapHoles) {  // host retries with smaller tiles
+            if (tid == 0) atomicOr(status, 1);
+            return;
+        }
+        if (tid == 0) {
+            s_meta[0] = tx0; s_meta[1] = ty0; s_meta[2] = tz0;
+            s_meta[3] = tx1 - tx0; s_meta[4] = ty1 - ty0; s_meta[5] = tz1 - tz0;
+            s_meta[6] = ng; s_meta[7] = nhg;
+        }
+        __syncthreads();
     }
 
-    const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
     const int S = p.S;
     const float inv_nv3 = 1.0f / (float)(p.nv * p.nv * p.nv);
     // crystal-d sub-point geometry
@@ -297,337 +390,411 @@ __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(3
     const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
     const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
 
-    for (int e = wave; e < nvox; e += kWaves) {
-        const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
-        const int ix = vx0 + lx, iy = vy0 + ly, izl = vz0 + lz;
-        float xc0, yc0, zc0;
-        voxel_centre(p, ix, iy, p.z0 + izl, xc0, yc0, zc0);
-        float xI, yI;
-        rotate_image(p, xc0, yc0, xI, yI);
-        const float zI = zc0;
+    for (int t = 0; t < uniform(s_meta[3] * s_meta[4] * s_meta[5]); ++t) {
+        const int gnx = uniform(s_meta[3]), gny = uniform(s_meta[4]);
+        const int tix = uniform(s_meta[0]) + t % gnx, tiy = uniform(s_meta[1]) + (t / gnx) % gny;
+        const int tiz = uniform(s_meta[2]) + t / (gnx * gny);
+        const int vx0 = tix * p.tx, vy0 = tiy * p.ty, vz0 = tiz * p.tz;
+        const int vx1 = min(vx0 + p.tx, p.X), vy1 = min(vy0 + p.ty, p.Y), vz1 = min(vz0 + p.tz, p.Zslab);
+        const int nvx = vx1 - vx0, nvy = vy1 - vy0;
+        const int nvox = nvx * nvy * (vz1 - vz0);
 
-        // -- exact cull predicate from the voxel centre (PESysMatGen.cu:506-589)
-        const float dist0 = sqrtf((D.cy - yI) * (D.cy - yI) + (D.cx - xI) * (D.cx - xI) + (D.cz - zI) * (D.cz - zI));
-        const float crit_self = D.rbound / dist0 / 2.0f;
-        const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
-
-        // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (crystal d)
-        const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - D.hy, Q_yhi = D.cy + D.hy;
-        const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
-        if (bundle_ok) {
-            // lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face; lanes >= 32 do z
-            const bool zaxis = lane >= 32;
-            const int v = lane & 31;
-            const float Pa = zaxis ? ((v & 1) ? zI + hvz : zI - hvz) : ((v & 1) ? xI + pex : xI - pex);
-            const float Py = (v & 2) ? P_yhi : P_ylo;
-            const float Qa = zaxis ? ((v & 4) ? D.cz + qez : D.cz - qez) : ((v & 4) ? D.cx + qex : D.cx - qex);
-            const float Qy = (v & 8) ? Q_yhi : Q_ylo;
-            for (int s = 0; s < p.n_slabs; ++s) {
-                const Slab SL = slb[s];
-                const float y = (v & 16) ? SL.y2 : SL.y1;
-                const float val = Pa + (Qa - Pa) * ((y - Py) / (Qy - Py));
-                const float mn = wave_min32(val), mx = wave_max32(val);
-                const float m = 2e-3f + 1e-5f * fmaxf(fabsf(mn), fabsf(mx));
-                if (lane == 0) { s_box[wave][s].x = mn - m; s_box[wave][s].y = mx + m; }
-                if (lane == 32) { s_box[wave][s].z = mn - m; s_box[wave][s].w = mx + m; }
+        // ---- tile cull: superset of every element's list ---------------------
+        int nc = uniform(s_meta[6]), nh = uniform(s_meta[7]);
+        if (grouped) {
+            nc = 0;
+            nh = 0;
+            float tcx, tcy, tcz, rho;
+            box_sphere(p, vx0, vy0, vz0, vx1, vy1, vz1, tcx, tcy, tcz, rho);
+            const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+            const int ng = uniform(s_meta[6]), nhg = uniform(s_meta[7]);
+            for (int base = 0; base < ng; base += kThreads) {
+                const int i = base + tid;
+                int k = 0;
+                bool take = false;
+                if (i < ng) {
+                    k = (int)s_grp[i];
+                    take = sphere_take_crystal(C, cry, k);
+                }
+                const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
+                if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
+            }
+            for (int base = 0; base < nhg; base += kThreads) {
+                const int i = base + tid;
+                int h = 0;
+                bool take = false;
+                if (i < nhg) {
+                    h = (int)s_hgrp[i];
+                    take = sphere_take_hole(C, hol[h]);
+                }
+                const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
+                if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
             }
         }
-        __builtin_amdgcn_wave_barrier();
+        if (nc > kCapCrystals || nh > kCapHoles) {  // host retries with smaller tiles
+            if (tid == 0) atomicOr(status, 1);
+            return;
+        }
+        __syncthreads();  // the candidate lists are complete before any wave reads them
+        const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
 
-        // -- crystals: bundle overlap, then the exact predicate; survivors are
-        //    kept as a bitmask and, when they fit, as compacted boxes
-        int n_att = 0;
-        bool any_rotated = false;  // some listed attenuator needs its own frame
-        for (int w = 0; w < ncw; w += 2) {
-            const int j = w * 32 + lane;
-            bool take = false;
-            Crystal K;
-            int k = 0;
-            if (j < nc) {
-                k = (int)s_cand[j];
-                K = cry[k];
-                take = true;
-                if (bundle_ok && K.slab >= 0.0f) {
-                    const float4 bb = s_box[wave][(int)K.slab];
-                    const float ex = (K.axis != 0.0f) ? K.hx : sqrtf(K.hx * K.hx + K.hz * K.hz);
-                    const float ez = (K.axis != 0.0f) ? K.hz : ex;
-                    take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
+        for (int e = wave; e < nvox; e += kWaves) {
+            const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
+            const int ix = vx0 + lx, iy = vy0 + ly, izl = vz0 + lz;
+            float xc0, yc0, zc0;
+            voxel_centre(p, ix, iy, p.z0 + izl, xc0, yc0, zc0);
+            float xI, yI;
+            rotate_image(p, xc0, yc0, xI, yI);
+            const float zI = zc0;
+
+            // -- exact cull predicate from the voxel centre (PESysMatGen.cu:506-589)
+            const float dist0 =
+                sqrtf((D.cy - yI) * (D.cy - yI) + (D.cx - xI) * (D.cx - xI) + (D.cz - zI) * (D.cz - zI));
+            const float crit_self = D.rbound / dist0 / 2.0f;
+            const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
+
+            // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (crystal d)
+            const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - D.hy, Q_yhi = D.cy + D.hy;
+            const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
+            if (bundle_ok) {
+                // lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face; lanes >= 32 do z.  The
+                // 2e-3 mm + 1e-5 relative margin covers the rounding of the vertex rays.
+                const bool zaxis = lane >= 32;
+                const int v = lane & 31;
+                const float Pa = zaxis ? ((v & 1) ? zI + hvz : zI - hvz) : ((v & 1) ? xI + pex : xI - pex);
+                const float Py = (v & 2) ? P_yhi : P_ylo;
+                const float Qa = zaxis ? ((v & 4) ? D.cz + qez : D.cz - qez) : ((v & 4) ? D.cx + qex : D.cx - qex);
+                const float Qy = (v & 8) ? Q_yhi : Q_ylo;
+                const float slope = (Qa - Pa) / (Qy - Py);
+                for (int s = 0; s < p.n_slabs; ++s) {
+                    const Slab SL = slb[s];
+                    const float y = (v & 16) ? SL.y2 : SL.y1;
+                    const float val = Pa + slope * (y - Py);
+                    const float rmn = row_min16(val), rmx = row_max16(val);
+                    const float mnx = fminf(lane_value(rmn, 0), lane_value(rmn, 16));
+                    const float mxx = fmaxf(lane_value(rmx, 0), lane_value(rmx, 16));
+                    const float mnz = fminf(lane_value(rmn, 32), lane_value(rmn, 48));
+                    const float mxz = fmaxf(lane_value(rmx, 32), lane_value(rmx, 48));
+                    const float mgx = 2e-3f + 1e-5f * fmaxf(fabsf(mnx), fabsf(mxx));
+                    const float mgz = 2e-3f + 1e-5f * fmaxf(fabsf(mnz), fabsf(mxz));
+                    if (lane == 0) s_box[wave][s] = make_float4(mnx - mgx, mxx + mgx, mnz - mgz, mxz + mgz);
                 }
+            }
+            __builtin_amdgcn_wave_barrier();
+
+            // -- crystals: bundle overlap, then the exact predicate; survivors are
+            //    kept as a bitmask and, when they fit, as compacted boxes
+            int n_att = 0;
+            bool any_rotated = false;  // some listed attenuator needs its own frame
+            for (int w = 0; w < ncw; w += 2) {
+                const int j = w * 32 + lane;
+                bool take = false;
+                Crystal K;
+                int k = 0;
+                if (j < nc) {
+                    k = (int)s_cand[j];
+                    K = cry[k];
+                    take = true;
+                    if (bundle_ok && K.slab >= 0.0f) {
+                        const float4 bb = s_box[wave][(int)K.slab];
+                        const float ex = (K.axis != 0.0f) ? K.hx : sqrtf(K.hx * K.hx + K.hz * K.hz);
+                        const float ez = (K.axis != 0.0f) ? K.hz : ex;
+                        take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
+                    }
+                    if (take) {
+                        const int fast =
+                            cull_fast(K.cx - xI, K.cy - yI, K.cz - zI, 0.5f * K.rbound, ux, uy, uz, crit_self);
+                        if (fast != 0) {
+                            take = fast > 0;
+                        } else {  // near the boundary: the reference's operations
+                            float crit = K.rbound;
+                            const float dk =
+                                sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) +
+                                      (zI - K.cz) * (zI - K.cz));
+                            crit = crit / dk / 2.0f;
+                            const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
+                            take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                                   crit + crit_self;
+                        }
+                    }
+                }
+                const unsigned long long bal = __ballot(take);
+                if (__ballot(take && K.axis == 0.0f)) any_rotated = true;
                 if (take) {
-                    const int fast = cull_fast(K.cx - xI, K.cy - yI, K.cz - zI, 0.5f * K.rbound, ux, uy, uz, crit_self);
-                    if (fast != 0) {
-                        take = fast > 0;
-                    } else {  // near the boundary: the reference's operations
-                        float crit = K.rbound;
-                        const float dk =
-                            sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) + (zI - K.cz) * (zI - K.cz));
-                        crit = crit / dk / 2.0f;
-                        const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
-                        take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                               crit + crit_self;
+                    const int slot = n_att + __popcll(bal & below);
+                    if (slot < kListCap) {
+                        Att A;
+                        A.x1 = K.cx - K.hx; A.x2 = K.cx + K.hx;
+                        A.y1 = K.cy - K.hy; A.y2 = K.cy + K.hy;
+                        A.z1 = K.cz - K.hz; A.z2 = K.cz + K.hz;
+                        A.mu = K.mu_t;
+                        A.k = (K.axis != 0.0f) ? k : -(k + 1);
+                        s_att[wave][slot] = A;
                     }
                 }
-            }
-            const unsigned long long bal = __ballot(take);
-            if (__ballot(take && K.axis == 0.0f)) any_rotated = true;
-            if (take) {
-                const int slot = n_att + __popcll(bal & below);
-                if (slot < kListCap) {
-                    Att A;
-                    A.x1 = K.cx - K.hx; A.x2 = K.cx + K.hx;
-                    A.y1 = K.cy - K.hy; A.y2 = K.cy + K.hy;
-                    A.z1 = K.cz - K.hz; A.z2 = K.cz + K.hz;
-                    A.mu = K.mu_t;
-                    A.k = (K.axis != 0.0f) ? k : -(k + 1);
-                    s_att[wave][slot] = A;
+                n_att += __popcll(bal);
+                if (lane == 0) {
+                    s_mask[wave][w] = (unsigned)bal;
+                    if (w + 1 < (kCapCrystals / 32)) s_mask[wave][w + 1] = (unsigned)(bal >> 32);
                 }
             }
-            n_att += __popcll(bal);
-            if (lane == 0) {
-                s_mask[wave][w] = (unsigned)bal;
-                if (w + 1 < (kCapCrystals / 32)) s_mask[wave][w + 1] = (unsigned)(bal >> 32);
-            }
-        }
-        // -- holes: bundle overlap, exact predicate; layer_holes bit m set when
-        //    some hole of layer m survives
-        unsigned layer_holes = 0;
-        int n_hol = 0;
-        for (int w = 0; w < nhw; w += 2) {
-            const int j = w * 32 + lane;
-            bool take = false;
-            int h = 0;
-            if (j < nh) {
-                h = (int)s_hcand[j];
-                const Hole H = hol[h];
-                take = true;
-                if (bundle_ok && H.slab >= 0.0f) {
-                    const float4 bb = s_box[wave][(int)H.slab];
-                    take = !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+            // -- holes: bundle overlap, exact predicate; layer_holes bit m set when
+            //    some hole of layer m survives
+            unsigned layer_holes = 0;
+            int n_hol = 0;
+            for (int w = 0; w < nhw; w += 2) {
+                const int j = w * 32 + lane;
+                bool take = false;
+                int h = 0;
+                if (j < nh) {
+                    h = (int)s_hcand[j];
+                    const Hole H = hol[h];
+                    take = true;
+                    if (bundle_ok && H.slab >= 0.0f) {
+                        const float4 bb = s_box[wave][(int)H.slab];
+                        take = !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+                    }
+                    if (take) {
+                        const float yh = (H.y1 + H.y2) / 2.0f;
+                        const int fast = cull_fast(H.x - xI, yh - yI, H.z - zI, H.rbound, ux, uy, uz, crit_self);
+                        if (fast != 0) {
+                            take = fast > 0;
+                        } else {  // near the boundary: the reference's operations
+                            float crit = H.rbound;
+                            const float dh =
+                                sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
+                            crit = crit / dh;
+                            const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
+                            take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                                   crit + crit_self;
+                        }
+                    }
+                    if (take) layer_holes |= 1u << (int)H.layer;
                 }
+                const unsigned long long bal = __ballot(take);
                 if (take) {
-                    const float yh = (H.y1 + H.y2) / 2.0f;
-                    const int fast = cull_fast(H.x - xI, yh - yI, H.z - zI, H.rbound, ux, uy, uz, crit_self);
-                    if (fast != 0) {
-                        take = fast > 0;
-                    } else {  // near the boundary: the reference's operations
-                        float crit = H.rbound;
-                        const float dh =
-                            sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
-                        crit = crit / dh;
-                        const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
-                        take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                               crit + crit_self;
-                    }
+                    const int slot = n_hol + __popcll(bal & below);
+                    if (slot < kHoleCap) s_hol[wave][slot] = h;
                 }
-                if (take) layer_holes |= 1u << (int)H.layer;
-            }
-            const unsigned long long bal = __ballot(take);
-            if (take) {
-                const int slot = n_hol + __popcll(bal & below);
-                if (slot < kHoleCap) s_hol[wave][slot] = h;
-            }
-            n_hol += __popcll(bal);
-            if (lane == 0) {
-                s_hmask[wave][w] = (unsigned)bal;
-                if (w + 1 < (kCapHoles / 32)) s_hmask[wave][w + 1] = (unsigned)(bal >> 32);
-            }
-        }
-#pragma unroll
-        for (int o = 32; o >= 1; o >>= 1) layer_holes |= (unsigned)__shfl_xor((int)layer_holes, o, 64);
-        layer_holes = uniform(layer_holes);
-        n_att = uniform(n_att);
-        n_hol = uniform(n_hol);
-        // -- plate classes: test per sub-ray, every sub-ray passes, every sub-ray misses
-        unsigned plate_test = 0;  // bit m: per-sub-ray test needed
-        bool element_blocked = false;
-        for (int m = 0; m < p.n_layers; ++m) {
-            const Plate PL = pla[m];
-            int cls = 0;
-            if (bundle_ok && PL.slab >= 0.0f) {
-                const float4 bb = s_box[wave][(int)PL.slab];
-                const bool crosses = (P_yhi < PL.y1 - 0.01f) && (Q_ylo > PL.y2 + 0.01f) && PL.thick >= 0.2f;
-                if (crosses && bb.x >= PL.x1 + 0.01f && bb.y <= PL.x2 - 0.01f && bb.z >= PL.z1 + 0.01f &&
-                    bb.w <= PL.z2 - 0.01f)
-                    cls = 1;
-                else if (bb.y < PL.x1 || bb.x > PL.x2 || bb.w < PL.z1 || bb.z > PL.z2)
-                    cls = 2;
-            }
-            if (cls == 0) plate_test |= 1u << m;
-            if (cls == 2) element_blocked = true;
-        }
-        __builtin_amdgcn_wave_barrier();
-
-        double acc = 0.0;
-        if (!element_blocked) {
-            // FP64 voxel centre (before rotation) for the collimator path
-            const double xc64 = ((double)ix - (double)p.X / 2.0 + 0.5) * (double)p.wx + (double)p.sx;
-            const double yc64 = ((double)iy - (double)p.Y / 2.0 + 0.5) * (double)p.wy + (double)p.sy;
-            const double zc64 = ((double)(p.z0 + izl) - (double)p.Zfull / 2.0 + 0.5) * (double)p.wz + (double)p.sz;
-            const bool att_fast = n_att <= kListCap;
-            const bool hol_fast = n_hol <= kHoleCap;
-            for (int base = 0; base < S; base += 64) {
-                const int s = base + lane;
-                if (s >= S) break;
-                // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
-                int ny, nx, nz;
-                int q = divmod(s, p.cy, rcy, ny);
-                q = divmod(q, p.cx, rcx, nx);
-                const int iv = divmod(q, p.cz, rcz, nz);
-                float xP = xI, yP = yI, zP = zI;
-                int va = 0, vb = 0, vc = 0;
-                if (p.nv > 1) {
-                    const int t1 = divmod(iv, p.nv, rnv, va);
-                    vc = divmod(t1, p.nv, rnv, vb);
-                    const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
-                    const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
-                    const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
-                    rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
-                    zP = zc0 + oz;
+                n_hol += __popcll(bal);
+                if (lane == 0) {
+                    s_hmask[wave][w] = (unsigned)bal;
+                    if (w + 1 < (kCapHoles / 32)) s_hmask[wave][w + 1] = (unsigned)(bal >> 32);
                 }
-                const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
-                const float yPs = yP - D.cy;
-                const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
-                // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
-                const float xDs = -wD / 2.0f + ((float)nx + 0.5f) * rcx * wD;
-                const float zDs = -hD / 2.0f + ((float)nz + 0.5f) * rcz * hD;
-                const float yDs = -tD / 2.0f + ((float)ny + 0.5f) * rcy * tD;
-                const float xQ = D.cx + (xDs * cD - zDs * sD);
-                const float zQ = D.cz + (xDs * sD + zDs * cD);
-                const float yQ = D.cy + yDs;
-
-                const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
-                const float dist = R.t;
-                // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
-                const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
-                const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
-                const float fy = A_y * fabsf(R.dy);
-                const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
-
-                bool blocked = false;
-                for (unsigned mm = plate_test; mm; mm &= mm - 1) {  // plate miss => blocked (PESysMatGen.cu:645-659)
-                    const int m = __builtin_ctz(mm);
-                    const Plate PL = pla[m];
-                    const float c = (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
-                                                 : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t,
-                                                                 R.pat, PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
-                    if (c < 0.1f) blocked = true;
+            }
+            {  // OR over the wave: one ballot per layer
+                unsigned any = 0;
+                for (int m = 0; m < p.n_layers; ++m)
+                    if (__ballot((layer_holes >> m) & 1u) != 0ull) any |= 1u << m;
+                layer_holes = uniform(any);
+            }
+            n_att = uniform(n_att);
+            n_hol = uniform(n_hol);
+            // -- plate classes: test per sub-ray, every sub-ray passes, every sub-ray misses
+            unsigned plate_test = 0;  // bit m: per-sub-ray test needed
+            bool element_blocked = false;
+            for (int m = 0; m < p.n_layers; ++m) {
+                const Plate PL = pla[m];
+                int cls = 0;
+                if (bundle_ok && PL.slab >= 0.0f) {
+                    const float4 bb = s_box[wave][(int)PL.slab];
+                    const bool crosses = (P_yhi < PL.y1 - 0.01f) && (Q_ylo > PL.y2 + 0.01f) && PL.thick >= 0.2f;
+                    if (crosses && bb.x >= PL.x1 + 0.01f && bb.y <= PL.x2 - 0.01f && bb.z >= PL.z1 + 0.01f &&
+                        bb.w <= PL.z2 - 0.01f)
+                        cls = 1;
+                    else if (bb.y < PL.x1 || bb.x > PL.x2 || bb.w < PL.z1 || bb.z > PL.z2)
+                        cls = 2;
                 }
-                float att = 0.0f;
-                double att64 = 0.0;
-                // collimator layers: plate length minus hole chords (PESysMatGen.cu:661-685)
-                if (layer_holes == 0) {
-                    const float icos = dist * frcp(R.dy);
-                    for (int m = 0; m < p.n_layers; ++m) {
-                        const Plate PL = pla[m];
-                        const float in_plate = PL.thick * icos;
-                        if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
-                    }
-                } else {
-                    // end points in FP64 with the same formulas
-                    double px = xc64, py = yc64, pz = zc64;
+                if (cls == 0) plate_test |= 1u << m;
+                if (cls == 2) element_blocked = true;
+            }
+            __builtin_amdgcn_wave_barrier();
+
+            double acc = 0.0;
+            if (!element_blocked) {
+                // FP64 voxel centre (before rotation) for the collimator path
+                const double xc64 = ((double)ix - (double)p.X / 2.0 + 0.5) * (double)p.wx + (double)p.sx;
+                const double yc64 = ((double)iy - (double)p.Y / 2.0 + 0.5) * (double)p.wy + (double)p.sy;
+                const double zc64 = ((double)(p.z0 + izl) - (double)p.Zfull / 2.0 + 0.5) * (double)p.wz + (double)p.sz;
+                const bool att_fast = n_att <= kListCap;
+                const bool hol_fast = n_hol <= kHoleCap;
+                for (int base = 0; base < S; base += 64) {
+                    const int s = base + lane;
+                    if (s >= S) break;
+                    // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
+                    int ny, nx, nz;
+                    int q = divmod(s, p.cy, rcy, ny);
+                    q = divmod(q, p.cx, rcx, nx);
+                    const int iv = divmod(q, p.cz, rcz, nz);
+                    float xP = xI, yP = yI, zP = zI;
+                    int va = 0, vb = 0, vc = 0;
                     if (p.nv > 1) {
-                        const double fnv = (double)p.nv;
-                        px += (((double)va + 0.5) / fnv - 0.5) * (double)p.wx;
-                        py += (((double)vb + 0.5) / fnv - 0.5) * (double)p.wy;
-                        pz += (((double)vc + 0.5) / fnv - 0.5) * (double)p.wz;
+                        const int t1 = divmod(iv, p.nv, rnv, va);
+                        vc = divmod(t1, p.nv, rnv, vb);
+                        const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
+                        const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
+                        const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
+                        rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
+                        zP = zc0 + oz;
                     }
-                    const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
-                    const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
-                    const double xs = -w64 / 2.0 + ((double)nx + 0.5) / (double)p.cx * w64;
-                    const double zs = -h64 / 2.0 + ((double)nz + 0.5) / (double)p.cz * h64;
-                    const double ys = -t64 / 2.0 + ((double)ny + 0.5) / (double)p.cy * t64;
-                    const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
-                    const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
-                    const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
-                    const Ray64 R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
-                    for (int m = 0; m < p.n_layers; ++m) {
+                    const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
+                    const float yPs = yP - D.cy;
+                    const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
+                    // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
+                    const float xDs = -wD / 2.0f + ((float)nx + 0.5f) * rcx * wD;
+                    const float zDs = -hD / 2.0f + ((float)nz + 0.5f) * rcz * hD;
+                    const float yDs = -tD / 2.0f + ((float)ny + 0.5f) * rcy * tD;
+                    const float xQ = D.cx + (xDs * cD - zDs * sD);
+                    const float zQ = D.cz + (xDs * sD + zDs * cD);
+                    const float yQ = D.cy + yDs;
+
+                    const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
+                    const float dist = R.t;
+                    // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
+                    const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
+                    const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
+                    const float fy = A_y * fabsf(R.dy);
+                    const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
+
+                    bool blocked = false;
+                    // plate miss => blocked (PESysMatGen.cu:645-659)
+                    for (unsigned mm = plate_test; mm; mm &= mm - 1) {
+                        const int m = __builtin_ctz(mm);
                         const Plate PL = pla[m];
-                        double holes_len = 0.0;
-                        if ((layer_holes >> m) & 1u) {
-                            auto one_hole = [&](int h) {
-                                const Hole H = hol[h];
-                                if ((int)H.layer != m) return;
-                                const double c = chord_hole64(R64, H.x, (double)H.y1_raw + (double)p.fov2c,
-                                                              (double)H.y2_raw + (double)p.fov2c, H.z, H.r);
-                                holes_len += c;
-                                att64 += (double)H.mu_t * c;
-                            };
-                            if (hol_fast) {
-                                for (int j = 0; j < n_hol; ++j) one_hole(uniform(s_hol[wave][j]));
-                            } else {
-                                for (int w = 0; w < nhw; ++w) {
-                                    unsigned msk = uniform(s_hmask[wave][w]);
-                                    while (msk) {
-                                        const int b = __builtin_ctz(msk);
-                                        msk &= msk - 1;
-                                        one_hole(uniform((int)s_hcand[w * 32 + b]));
+                        const float c =
+                            (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
+                                         : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t, R.pat,
+                                                         PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
+                        if (c < 0.1f) blocked = true;
+                    }
+                    float att = 0.0f;
+                    double att64 = 0.0;
+                    // collimator layers: plate length minus hole chords (PESysMatGen.cu:661-685)
+                    if (layer_holes == 0) {
+                        const float icos = dist * frcp(R.dy);
+                        for (int m = 0; m < p.n_layers; ++m) {
+                            const Plate PL = pla[m];
+                            const float in_plate = PL.thick * icos;
+                            if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
+                        }
+                    } else {
+                        // end points in FP64 with the same formulas
+                        double px = xc64, py = yc64, pz = zc64;
+                        if (p.nv > 1) {
+                            const double fnv = (double)p.nv;
+                            px += (((double)va + 0.5) / fnv - 0.5) * (double)p.wx;
+                            py += (((double)vb + 0.5) / fnv - 0.5) * (double)p.wy;
+                            pz += (((double)vc + 0.5) / fnv - 0.5) * (double)p.wz;
+                        }
+                        const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
+                        const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
+                        const double xs = -w64 / 2.0 + ((double)nx + 0.5) / (double)p.cx * w64;
+                        const double zs = -h64 / 2.0 + ((double)nz + 0.5) / (double)p.cz * h64;
+                        const double ys = -t64 / 2.0 + ((double)ny + 0.5) / (double)p.cy * t64;
+                        const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
+                        const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
+                        const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
+                        const Ray64 R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
+                        for (int m = 0; m < p.n_layers; ++m) {
+                            const Plate PL = pla[m];
+                            double holes_len = 0.0;
+                            if ((layer_holes >> m) & 1u) {
+                                auto one_hole = [&](int h) {
+                                    const Hole H = hol[h];
+                                    if ((int)H.layer != m) return;
+                                    const double c = chord_hole64(R64, H.x, (double)H.y1_raw + (double)p.fov2c,
+                                                                  (double)H.y2_raw + (double)p.fov2c, H.z, H.r);
+                                    holes_len += c;
+                                    att64 += (double)H.mu_t * c;
+                                };
+                                if (hol_fast) {
+                                    for (int j = 0; j < n_hol; ++j) one_hole(uniform(s_hol[wave][j]));
+                                } else {
+                                    for (int w = 0; w < nhw; ++w) {
+                                        unsigned msk = uniform(s_hmask[wave][w]);
+                                        while (msk) {
+                                            const int b = __builtin_ctz(msk);
+                                            msk &= msk - 1;
+                                            one_hole(uniform((int)s_hcand[w * 32 + b]));
+                                        }
                                     }
                                 }
                             }
+                            const double in_plate = (double)PL.thick / R64.ky;
+                            if ((in_plate - holes_len) >= (double)0.00001f)
+                                att64 += (double)PL.mu_t * (in_plate - holes_len);
                         }
-                        const double in_plate = (double)PL.thick / R64.ky;
-                        if ((in_plate - holes_len) >= (double)0.00001f)
-                            att64 += (double)PL.mu_t * (in_plate - holes_len);
                     }
-                }
-                // other crystals on the way (PESysMatGen.cu:687-725)
-                if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
-                    // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
-                    // wave; unrolled so the LDS reads of several boxes are in flight together
+                    // other crystals on the way (PESysMatGen.cu:687-725)
+                    if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
+                        // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
+                        // wave; unrolled so the LDS reads of several boxes are in flight together
 #pragma unroll 4
-                    for (int j = 0; j < n_att; ++j) {
-                        const Att A = s_att[wave][j];
-                        att += chord_world(R, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
-                    }
-                } else if (att_fast) {
-                    for (int j = 0; j < n_att; ++j) {
-                        const Att A = s_att[wave][j];
-                        const int k = uniform(A.k);
-                        const bool rotated = k < 0;
-                        att += crystal_chord(R, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2,
-                                             rotated) * A.mu;
-                    }
-                } else {
-                    for (int w = 0; w < ncw; ++w) {
-                        unsigned msk = uniform(s_mask[wave][w]);
-                        while (msk) {
-                            const int b = __builtin_ctz(msk);
-                            msk &= msk - 1;
-                            const int k = uniform((int)s_cand[w * 32 + b]);
-                            const Crystal K = cry[k];
-                            att += crystal_chord(R, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy, K.cy + K.hy,
-                                                 K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) * K.mu_t;
+                        for (int j = 0; j < n_att; ++j) {
+                            const Att A = s_att[wave][j];
+                            att += chord_world(R, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
+                        }
+                    } else if (att_fast) {
+                        for (int j = 0; j < n_att; ++j) {
+                            const Att A = s_att[wave][j];
+                            const int k = uniform(A.k);
+                            const bool rotated = k < 0;
+                            att += crystal_chord(R, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2,
+                                                 rotated) * A.mu;
+                        }
+                    } else {
+                        for (int w = 0; w < ncw; ++w) {
+                            unsigned msk = uniform(s_mask[wave][w]);
+                            while (msk) {
+                                const int b = __builtin_ctz(msk);
+                                msk &= msk - 1;
+                                const int k = uniform((int)s_cand[w * 32 + b]);
+                                const Crystal K = cry[k];
+                                att += crystal_chord(R, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy, K.cy + K.hy,
+                                                     K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) * K.mu_t;
+                            }
                         }
                     }
+                    // path inside crystal d before the sub-cell, and absorption in the
+                    // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
+                    const float ux1 = ((float)nx * rcx - 0.5f) * wD, ux2 = (((float)nx + 1.0f) * rcx - 0.5f) * wD;
+                    const float uy1 = ((float)ny * rcy - 0.5f) * tD, uy2 = (((float)ny + 1.0f) * rcy - 0.5f) * tD;
+                    const float uz1 = ((float)nz * rcz - 0.5f) * hD, uz2 = (((float)nz + 1.0f) * rcz - 0.5f) * hD;
+                    const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
+                    const float a1 = open_q(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz);
+                    const float a2 = open_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
+                    att += (a1 - a2) * D.mu_t;
+                    const float eff = expf(-(float)(att64 + (double)att));
+                    const float l_abs = absorb_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
+                    const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
+                    const float contrib = eff * solid * absorb;
+                    if (!blocked) acc += (double)contrib;
                 }
-                // path inside crystal d before the sub-cell, and absorption in the
-                // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
-                const float ux1 = ((float)nx * rcx - 0.5f) * wD, ux2 = (((float)nx + 1.0f) * rcx - 0.5f) * wD;
-                const float uy1 = ((float)ny * rcy - 0.5f) * tD, uy2 = (((float)ny + 1.0f) * rcy - 0.5f) * tD;
-                const float uz1 = ((float)nz * rcz - 0.5f) * hD, uz2 = (((float)nz + 1.0f) * rcz - 0.5f) * hD;
-                const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
-                const float a1 = open_q(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz);
-                const float a2 = open_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
-                att += (a1 - a2) * D.mu_t;
-                const float eff = expf(-(float)(att64 + (double)att));
-                const float l_abs = absorb_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
-                const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
-                const float contrib = eff * solid * absorb;
-                if (!blocked) acc += (double)contrib;
             }
+            acc = wave_sum64(acc);
+            if (lane == 0) {
+                row[(long long)izl * p.X * p.Y + iy * p.X + ix] = (float)(acc * (double)inv_nv3);
+            }
+            __builtin_amdgcn_wave_barrier();
         }
-        acc = wave_sum64(acc);
-        if (lane == 0) {
-            row[(long long)izl * p.X * p.Y + iy * p.X + ix] = (float)(acc * (double)inv_nv3);
-        }
-        __builtin_amdgcn_wave_barrier();
+        __syncthreads();  // every wave is done with this tile's lists before the next tile rewrites them
     }
 }
 
-// Host launcher: grid = detectors x tiles, one workgroup each.
-hipError_t launch_pe(const PeParams& p, const Crystal* cry, const Hole* hol, const Plate* pla, const Slab* slb,
-                     float* dst, int* status, hipStream_t stream) {
-    const long long tiles = (long long)p.ntx * p.nty * p.ntz;
+// Host launcher: grid = detectors x tile groups, one workgroup each.  Tile
+// groups share one cull over the detector list; a launch that would have too
+// few workgroups runs one tile per workgroup instead.  Either way every
+// element's result is the same.
+hipError_t launch_pe(const PeParams& p0, const Crystal* cry, const Hole* hol, const Plate* pla, const Slab* slb,
+                     const PeLists& L, float* dst, int* status, hipStream_t stream) {
+    PeParams p = p0;
+    if ((long long)p.ngx * p.ngy * p.ngz * p.det_count < kGroupMinWorkgroups) {
+        p.gx = p.gy = p.gz = 1;
+        p.ngx = p.ntx;
+        p.ngy = p.nty;
+        p.ngz = p.ntz;
+    }
+    const long long tiles = (long long)p.ngx * p.ngy * p.ngz;
     if (tiles <= 0 || p.det_count <= 0) return hipSuccess;
     // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
     const long long max_blocks = (1LL << 32) / kThreads - 1;
@@ -637,11 +804,23 @@ hipError_t launch_pe(const PeParams& p, const Crystal* cry, const Hole* hol, con
         q.det_begin = p.det_begin + d0;
         q.det_count = std::min(chunk, p.det_count - d0);
         hipLaunchKernelGGL(pe_tile_kernel, dim3((unsigned)(tiles * q.det_count)), dim3(kThreads), 0, stream, q, cry,
-                           hol, pla, slb, dst + (long long)d0 * p.row_stride, status);
+                           hol, pla, slb, L, dst + (long long)d0 * p.row_stride, status);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
     }
     return hipSuccess;
+}
+
+}  // namespace sysmat
+
+namespace sysmat {
+
+hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, int fill, int* counts,
+                                const PeLists& L, hipStream_t stream) {
+    if (p.n_det <= 0) return hipSuccess;
+    hipLaunchKernelGGL(pe_detector_cull_kernel, dim3((unsigned)p.n_det), dim3(kThreads), 0, stream, p, cry, hol, fill,
+                       counts, L);
+    return hipGetLastError();
 }
 
 }  // namespace sysmat

@@ -8,9 +8,16 @@ namespace sysmat {
 
 constexpr int kCapCrystals = 4096;  // tile candidate crystals (reference per-element cap: 3000, PESysMatGen.cu:481)
 constexpr int kCapHoles = 1024;     // tile candidate holes (reference: 1000 per layer, PESysMatGen.cu:485)
+constexpr int kCapGroup = 4096;     // tile-group candidate crystals kept in LDS (beyond: retry with smaller tiles)
+static_assert(kCapGroup <= kCapCrystals, "a one-tile group writes its crystal list into the tile list");
+constexpr int kGroupTiles = 2;      // tiles per group along each axis
+// Plans with fewer (detector, tile) pairs run one tile per workgroup (finer work
+// units, shorter tail); so do launches that would have fewer workgroups.
+constexpr long long kGroupMinTiles = 1LL << 18;
+constexpr long long kGroupMinWorkgroups = 2048;
 // Revision of the tracer's schedule; part of the autotuner's cache key, so tile
 // choices tuned for an older schedule are not reused.
-constexpr int kPeScheduleRevision = 2;
+constexpr int kPeScheduleRevision = 3;
 
 struct PeParams {
     // image slab (in-band Img[2], Img[30], Img[31] of the reference ABI)
@@ -23,6 +30,7 @@ struct PeParams {
     int S;                     // sub-rays per element = nv^3 * cx * cy * cz
     // tiling
     int tx, ty, tz, ntx, nty, ntz;
+    int gx, gy, gz, ngx, ngy, ngz;  // tiles per workgroup group, group counts
     // detector rows of this launch
     int n_det, det_begin, det_count;
     int n_layers, n_holes, n_slabs;
@@ -33,7 +41,21 @@ struct PeParams {
     float fov2c;               // Img[11]
 };
 
+// Per-detector candidate lists (CSR over detectors, indices in 16 bits): the
+// crystals and holes that may attenuate some sub-ray from the image slab to the
+// detector; built once per plan by launch_detector_cull.
+struct PeLists {
+    const long long* cry_ptr;   // [n_det + 1]
+    unsigned short* cry_list;
+    const long long* hol_ptr;   // [n_det + 1]
+    unsigned short* hol_list;
+};
+
 hipError_t launch_pe(const PeParams& p, const Crystal* cry, const Hole* hol, const Plate* pla, const Slab* slb,
-                     float* dst, int* status, hipStream_t stream);
+                     const PeLists& L, float* dst, int* status, hipStream_t stream);
+// fill = 0: counts[2 d] / counts[2 d + 1] = list lengths of detector d; fill = 1:
+// write the lists at L's offsets.  One workgroup per detector of the table.
+hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, int fill, int* counts,
+                                const PeLists& L, hipStream_t stream);
 
 }  // namespace sysmat

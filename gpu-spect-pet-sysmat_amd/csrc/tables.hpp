@@ -121,6 +121,9 @@ inline std::string build_tables(const float* col, const float* det, float fov2c,
         T.plates.push_back(p);
     }
     T.n_holes = h;
+    // the tracer keeps candidate crystal / hole indices in 16 bits
+    if (T.n_det > 65535) return "more than 65535 detector crystals";
+    if (T.n_holes > 65535) return "more than 65535 collimator holes";
     for (int d = 0; d < T.n_det; ++d) {
         const float* D = det + 12 * d;
         Crystal c{};

@@ -85,6 +85,18 @@ def test_tiles_and_determinism(lib):
     assert np.array_equal(a, b) and np.array_equal(a, c) and np.array_equal(a, d)
 
 
+def test_overflowing_tile_groups_retry_bit_identically(lib):
+    """16384 crystals (4 layers of 64 x 64): with 16^3-voxel tiles (an 8 mm cube) the
+    candidate list of a back-layer detector exceeds the 4096 entries kept per workgroup
+    (about 5000 by the sphere bound), so the launch retries with halved tiles; the rows
+    must equal the 4^3-tile rows bit for bit."""
+    g = params.config2(seed=0)
+    rows = (12288 + 2080, 12288 + 2084)
+    a = pe_compute(g, sub=(1, 1, 1), det_range=rows, tile=(16, 16, 16))
+    b = pe_compute(g, sub=(1, 1, 1), det_range=rows, tile=(4, 4, 4))
+    assert np.array_equal(a, b) and a.max() > 0
+
+
 def test_bundle_filter_changes_nothing(lib):
     g = params.tiny(seed=5, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3, n_plates=2)
     a = pe_compute(g, bundle_filter=True)
