@@ -85,7 +85,9 @@ def sync_t():
     return time.perf_counter()
 
 
-def pe_full(geo, nv, sub, dev, block=2048):
+def pe_full(geo, nv, sub, dev, block=2048, log=None):
+    """Every PE row of ``geo`` into one HBM-resident (N, V) tensor; with ``log`` each row
+    block is synchronised and reported (long runs)."""
     import torch
     from sysmat.pe import PePlan
     N, V = geo.n_det, geo.n_vox
@@ -94,6 +96,9 @@ def pe_full(geo, nv, sub, dev, block=2048):
     t0 = sync_t()
     for d0 in range(0, N, block):
         plan.launch(d0, min(block, N - d0), pe[d0:])
+        if log is not None:
+            plan.sync()
+            log(f"  PE rows {d0 + min(block, N - d0)}/{N} {sync_t() - t0:.1f}s")
     plan.sync()
     t = sync_t() - t0
     plan.close()
@@ -231,26 +236,33 @@ def scatter_c4(dev, log):
     return res
 
 
-def scatter_c5(dev, log, sample_rows=512, apply_block=2048):
-    """Surrogate geometry (16384 detectors in 4 layers, image reduced to 32^3 x 1.6 mm):
-    PE, the collimator scatter, surrogate training and the surrogate apply (every row,
+def scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_block=2048, tag="C5r"):
+    """Surrogate geometry (16384 detectors in 4 layers; image ``image``^3 voxels of ``voxel``
+    mm): PE, the collimator scatter, surrogate training and the surrogate apply (every row,
     in blocks of ``apply_block`` rows) measured in full; the exact crystal scatter timed
-    on a seeded row sample and projected linearly to all rows."""
+    on a seeded row sample and projected linearly to all rows.  The PE matrix stays
+    resident (the surrogate reads the rows of the scatter sites); the collimator scatter
+    and the surrogate output are produced block by block into reused buffers."""
     import numpy as np
     import torch
     from sysmat import params
     from sysmat.scatter import ScatterPlan
     from sysmat.surrogate import ScatterSurrogate, TrainConfig, usable_rows
     g = params.config5()
-    g.img[0:3] = 32
-    g.img[3:6] = 1.6
+    g.img[0:3] = image
+    g.img[3:6] = voxel
     N, V = g.n_det, g.n_vox
-    pe, t_pe = pe_full(g, 1, (2, 2, 2), dev)
-    log(f"C5r PE {t_pe:.2f}s")
+    long_run = V * N > (1 << 32)
+    pe, t_pe = pe_full(g, 1, (2, 2, 2), dev, log=log if long_run else None)
+    log(f"{tag} PE {t_pe:.2f}s")
     plan = ScatterPlan(g, bitmaps=False)
-    col = torch.empty((N, V), dtype=torch.float32, device=dev)
+    blk = torch.empty((apply_block, V), dtype=torch.float32, device=dev)
     t0 = sync_t()
-    plan.collimator(0, N, col)
+    for b in range(0, N, apply_block):
+        plan.collimator(b, min(apply_block, N - b), blk)
+        if long_run:
+            torch.cuda.synchronize()
+            log(f"  collimator rows {b + min(apply_block, N - b)}/{N} {sync_t() - t0:.1f}s")
     t_col = sync_t() - t0
     rng = np.random.default_rng(123)
     rows = np.sort(rng.choice(usable_rows(g), sample_rows, replace=False))
@@ -258,20 +270,24 @@ def scatter_c5(dev, log, sample_rows=512, apply_block=2048):
     t0 = sync_t()
     plan.crystal_rows(pe, rows, ex)
     t_ex = sync_t() - t0
-    log(f"C5r exact crystal {sample_rows} rows {t_ex:.2f}s, collimator all rows {t_col:.2f}s")
+    del ex
+    log(f"{tag} exact crystal {sample_rows} rows {t_ex:.2f}s, collimator all rows {t_col:.2f}s")
     sur = ScatterSurrogate(g, use_bitmap=False)
     t0 = sync_t()
     tr = sur.train(pe, TrainConfig(seed=0))
     t_train = sync_t() - t0
+    log(f"{tag} surrogate trained {t_train:.2f}s")
     judge = sur.judge(pe)
+    log(f"{tag} judge rows done")
     # surrogate apply (near field + MLP + exact fallback) over every row, in row blocks
-    out = torch.empty((apply_block, V), dtype=torch.float32, device=dev)
     n_fb = 0
     t0 = sync_t()
     for b in range(0, N, apply_block):
-        n_fb += sur.apply(pe, b, min(apply_block, N - b), out)
+        n_fb += sur.apply(pe, b, min(apply_block, N - b), blk)
+        if long_run:
+            log(f"  surrogate rows {b + min(apply_block, N - b)}/{N} {sync_t() - t0:.1f}s")
     t_ap = sync_t() - t0
-    log(f"C5r surrogate apply all rows {t_ap:.2f}s (train {t_train:.2f}s)")
+    log(f"{tag} surrogate apply all rows {t_ap:.2f}s (train {t_train:.2f}s)")
     per_row_exact = t_ex / sample_rows
     t_cry_full = per_row_exact * N
     t_sur_full = t_ap + t_train
@@ -281,10 +297,13 @@ def scatter_c5(dev, log, sample_rows=512, apply_block=2048):
     # total-sum error on the judge rows for the whole system matrix (PE + collimator + crystal)
     jr = sur.judge_rows
     jpe = pe[torch.as_tensor(jr, device=dev).long()].double().sum().item()
-    jcol = col[torch.as_tensor(jr, device=dev).long()].double().sum().item()
+    jcol = 0.0
+    for r in jr:
+        plan.collimator(int(r), 1, blk)
+        jcol += blk[0].double().sum().item()
     res = {
-        "workload": "surrogate geometry: 4 x 64x64 crystals (16384 det), 1 W plate 6x6 holes, image 32^3 x 1.6 mm, "
-                    "PE sampling (1, 2x2x2)",
+        "workload": f"surrogate geometry: 4 x 64x64 crystals (16384 det), 1 W plate 6x6 holes, image "
+                    f"{image}^3 x {voxel} mm, PE sampling (1, 2x2x2)",
         "elements": elems,
         "projection": f"exact crystal scatter timed on {sample_rows} seeded rows with a contribution and "
                       f"scaled to {N} rows; PE, collimator scatter, training and the surrogate apply "
@@ -308,6 +327,8 @@ def scatter_c5(dev, log, sample_rows=512, apply_block=2048):
     res["system_matrix_total_sum_rel_err_judge_rows"] = round(cry_tot * jcry / (jpe + jcol + jcry), 7)
     plan.close()
     sur.close()
+    del pe, blk, jex
+    torch.cuda.empty_cache()
     return res
 
 
@@ -320,6 +341,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scatter", action="store_true", help="skip the PE+scatter legs")
     ap.add_argument("--c4", action="store_true", help="also run the batched collimator-variant leg (minutes)")
+    ap.add_argument("--c5-full", action="store_true",
+                    help="also run the surrogate leg on the full 128^3 x 0.4 mm image (about 12 minutes, 150 GB HBM)")
     ap.add_argument("--cpu-rows", type=int, default=0, help="CPU baseline rows (0: one per host core, <= 64)")
     ap.add_argument("--calib-copy", action="store_true",
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3)")
@@ -447,6 +470,17 @@ def main():
             pe_scatter["c4"] = scatter_c4(dev, log)
         except Exception as e:
             pe_scatter["c4"] = {"error": repr(e)}
+    if args.c5_full:
+        if pe_scatter is None:
+            plan.close()
+            del out
+            torch.cuda.empty_cache()
+            pe_scatter = {}
+        try:
+            pe_scatter["c5_full"] = scatter_c5(dev, log, image=128, voxel=0.4, sample_rows=64, apply_block=1024,
+                                               tag="C5")
+        except Exception as e:
+            pe_scatter["c5_full"] = {"error": repr(e)}
 
     cpu = None
     if not args.no_cpu:

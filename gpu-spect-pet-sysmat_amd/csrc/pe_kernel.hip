@@ -383,8 +383,10 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
     const float absorb_scale = D.mu_pe / D.mu_t;
-    // world AABB of crystal d (bundle filter end box)
-    const float qex = fabsf(cD) * D.hx + fabsf(sD) * D.hz, qez = fabsf(sD) * D.hx + fabsf(cD) * D.hz;
+    // world AABB of the sub-cell centres of crystal d (bundle filter end box): the
+    // centres lie within (w/2)(1 - 1/n) of the crystal centre along each crystal axis
+    const float qhx = D.hx * (1.0f - rcx), qhy = D.hy * (1.0f - rcy), qhz = D.hz * (1.0f - rcz);
+    const float qex = fabsf(cD) * qhx + fabsf(sD) * qhz, qez = fabsf(sD) * qhx + fabsf(cD) * qhz;
     const float hvx = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
     const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
     const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
@@ -453,12 +455,16 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             const float crit_self = D.rbound / dist0 / 2.0f;
             const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
 
-            // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (crystal d)
-            const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - D.hy, Q_yhi = D.cy + D.hy;
+            // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
+            const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - qhy, Q_yhi = D.cy + qhy;
             const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
             if (bundle_ok) {
-                // lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face; lanes >= 32 do z.  The
-                // 2e-3 mm + 1e-5 relative margin covers the rounding of the vertex rays.
+                // lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face; lanes >= 32 do z.  Each vertex
+                // ray is taken at the slab face clamped to its own y-range [Py, Qy] (a segment
+                // covers no other y): the lateral position a + (b - a) clamp(t, 0, 1) is monotone
+                // in each end-point coordinate, so the 32 vertex rays still bound every sub-ray
+                // segment in the slab.  The 2e-3 mm + 1e-5 relative margin covers the rounding
+                // of the vertex rays and of the sub-point coordinates.
                 const bool zaxis = lane >= 32;
                 const int v = lane & 31;
                 const float Pa = zaxis ? ((v & 1) ? zI + hvz : zI - hvz) : ((v & 1) ? xI + pex : xI - pex);
@@ -469,7 +475,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                 for (int s = 0; s < p.n_slabs; ++s) {
                     const Slab SL = slb[s];
                     const float y = (v & 16) ? SL.y2 : SL.y1;
-                    const float val = Pa + slope * (y - Py);
+                    const float val = Pa + slope * (fminf(fmaxf(y, Py), Qy) - Py);
                     const float rmn = row_min16(val), rmx = row_max16(val);
                     const float mnx = fminf(lane_value(rmn, 0), lane_value(rmn, 16));
                     const float mxx = fmaxf(lane_value(rmx, 0), lane_value(rmx, 16));
