@@ -215,6 +215,89 @@ __device__ inline void box_sphere(const PeParams& p, int vx0, int vy0, int vz0, 
     rho = 0.5f * sqrtf((bx - ax) * (bx - ax) + (by - ay) * (by - ay) + (bz - az) * (bz - az)) * 1.0001f + 1e-4f;
 }
 
+// Rotated-frame centre and half extents of the voxel centres of the box [v0, v1)
+// (slab-local z).
+__device__ inline void box_extent(const PeParams& p, int vx0, int vy0, int vz0, int vx1, int vy1, int vz1, float& cx,
+                                  float& cy, float& cz, float& hx, float& hy, float& hz) {
+    float ax, ay, az, bx, by, bz;
+    voxel_centre(p, vx0, vy0, p.z0 + vz0, ax, ay, az);
+    voxel_centre(p, vx1 - 1, vy1 - 1, p.z0 + vz1 - 1, bx, by, bz);
+    rotate_image(p, 0.5f * (ax + bx), 0.5f * (ay + by), cx, cy);
+    cz = 0.5f * (az + bz);
+    const float ux = 0.5f * (bx - ax), uy = 0.5f * (by - ay);
+    hx = fabsf(p.rc) * ux + fabsf(p.rs) * uy;
+    hy = fabsf(p.rs) * ux + fabsf(p.rc) * uy;
+    hz = 0.5f * (bz - az);
+}
+
+// Lateral (x, z) bounds, per y-slab, of every segment from a point of box P
+// (centre pc, half extents ph) to a point of box Q (centre qc, half extents qh),
+// for P entirely before Q in y.  Lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face;
+// lanes >= 32 do z.  Each of the 32 vertex rays is taken at the slab face
+// clamped to its own y-range [Py, Qy] (a segment covers no other y): the lateral
+// position a + (b - a) clamp(t, 0, 1) is monotone in each end-point coordinate,
+// so the vertex rays bound every segment in the slab.  The 2e-3 mm + 1e-5
+// relative margin covers the rounding of the vertex rays and of the points.
+// Lane 0 writes box[s] = (x lo, x hi, z lo, z hi); the whole wave calls this.
+// A crystal or hole of slab s outside box[s] has a zero chord on every segment.
+__device__ inline void bundle_boxes(const PeParams& p, const Slab* __restrict__ slb, int lane, float pcx, float pcy,
+                                    float pcz, float phx, float phy, float phz, float qcx, float qcy, float qcz,
+                                    float qhx, float qhy, float qhz, float4* box) {
+    const bool zaxis = lane >= 32;
+    const int v = lane & 31;
+    const float Pa = zaxis ? ((v & 1) ? pcz + phz : pcz - phz) : ((v & 1) ? pcx + phx : pcx - phx);
+    const float Py = (v & 2) ? pcy + phy : pcy - phy;
+    const float Qa = zaxis ? ((v & 4) ? qcz + qhz : qcz - qhz) : ((v & 4) ? qcx + qhx : qcx - qhx);
+    const float Qy = (v & 8) ? qcy + qhy : qcy - qhy;
+    const float slope = (Qa - Pa) / (Qy - Py);
+    for (int s = 0; s < p.n_slabs; ++s) {
+        const Slab SL = slb[s];
+        const float y = (v & 16) ? SL.y2 : SL.y1;
+        const float val = Pa + slope * (fminf(fmaxf(y, Py), Qy) - Py);
+        const float rmn = row_min16(val), rmx = row_max16(val);
+        const float mnx = fminf(lane_value(rmn, 0), lane_value(rmn, 16));
+        const float mxx = fmaxf(lane_value(rmx, 0), lane_value(rmx, 16));
+        const float mnz = fminf(lane_value(rmn, 32), lane_value(rmn, 48));
+        const float mxz = fmaxf(lane_value(rmx, 32), lane_value(rmx, 48));
+        const float mgx = 2e-3f + 1e-5f * fmaxf(fabsf(mnx), fabsf(mxx));
+        const float mgz = 2e-3f + 1e-5f * fmaxf(fabsf(mnz), fabsf(mxz));
+        if (lane == 0) box[s] = make_float4(mnx - mgx, mxx + mgx, mnz - mgz, mxz + mgz);
+    }
+}
+
+// Crystal k / hole H against the bundle boxes (true: may have a non-zero chord).
+__device__ inline bool bundle_take_crystal(const Crystal* __restrict__ cry, int k, const float4* box) {
+    const float4 c0 = reinterpret_cast<const float4*>(cry + k)[0];  // cx, cy, cz, mu_t
+    const float4 c1 = reinterpret_cast<const float4*>(cry + k)[1];  // hx, hy, hz, rot
+    const float4 c3 = reinterpret_cast<const float4*>(cry + k)[3];  // mu_c, res, slab, axis
+    if (c3.z < 0.0f) return true;
+    const float4 bb = box[(int)c3.z];
+    const float ex = (c3.w != 0.0f) ? c1.x : sqrtf(c1.x * c1.x + c1.z * c1.z);
+    const float ez = (c3.w != 0.0f) ? c1.z : ex;
+    return !(c0.x + ex < bb.x || c0.x - ex > bb.y || c0.z + ez < bb.z || c0.z - ez > bb.w);
+}
+__device__ inline bool bundle_take_hole(const Hole& H, const float4* box) {
+    if (H.slab < 0.0f) return true;
+    const float4 bb = box[(int)H.slab];
+    return !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+}
+
+// Bundle boxes of every sub-ray from the voxel sub-points of the box [v0, v1) to
+// the sub-cell centres of crystal D (half extents pex/pey/pez, qex/qhy/qez) into
+// box, computed by wave 0; returns whether they apply (P entirely more than 1 mm
+// before Q in y).  Every thread of the block calls it (one barrier).
+__device__ inline bool group_bundle(const PeParams& p, const Slab* __restrict__ slb, const Crystal& D, int vx0,
+                                    int vy0, int vz0, int vx1, int vy1, int vz1, float pex, float pey, float pez,
+                                    float qex, float qhy, float qez, int lane, int wave, float4* box) {
+    float cx, cy, cz, hx, hy, hz;
+    box_extent(p, vx0, vy0, vz0, vx1, vy1, vz1, cx, cy, cz, hx, hy, hz);
+    const bool ok = p.use_bundle && ((D.cy - qhy) - (cy + hy + pey) > 1.0f);
+    if (ok && wave == 0)
+        bundle_boxes(p, slb, lane, cx, cy, cz, hx + pex, hy + pey, hz + pez, D.cx, D.cy, D.cz, qex, qhy, qez, box);
+    __syncthreads();
+    return ok;
+}
+
 // Ordered block-wide compaction step: every thread offers `take`; returns the
 // slot of a taken item (from `base_count`) and adds the block's count to it.
 // Needs s_wcount[kWaves] and all threads of the block.
@@ -287,6 +370,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     __shared__ Att s_att[kWaves][kListCap];
     __shared__ int s_hol[kWaves][kHoleCap];
     __shared__ float4 s_box[kWaves][kMaxSlabs];
+    __shared__ float4 s_tbox[kMaxSlabs];  // bundle boxes of the tile group, then of each tile
     __shared__ int s_wcount[kWaves];
     __shared__ int s_meta[8];  // tile-group state, re-read per tile (keeps it out of scalar registers)
 
@@ -319,58 +403,6 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
         return;
     }
 
-    // ---- group cull: the group's voxel-centre sphere over the detector's
-    //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
-    //      then culls these shorter lists (a group of one tile writes its tile
-    //      lists directly).  Ordered compaction keeps every list sorted by index.
-    const bool grouped = (tx1 - tx0) * (ty1 - ty0) * (tz1 - tz0) > 1;
-    {
-        int ng = 0, nhg = 0;
-        float tcx, tcy, tcz, rho;
-        box_sphere(p, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, tcx, tcy, tcz, rho);
-        const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
-        const long long c0 = L.cry_ptr[d], c1 = L.cry_ptr[d + 1];
-        for (long long base = c0; base < c1; base += kThreads) {
-            const long long i = base + tid;
-            int k = 0;
-            bool take = false;
-            if (i < c1) {
-                k = (int)L.cry_list[i];
-                take = sphere_take_crystal(C, cry, k);
-            }
-            const int slot = block_compact(take, ng, s_wcount, lane, wave, below);
-            if (take && slot < kCapGroup) {
-                if (grouped) s_grp[slot] = (unsigned short)k;
-                else s_cand[slot] = (unsigned short)k;
-            }
-        }
-        const long long h0 = L.hol_ptr[d], h1 = L.hol_ptr[d + 1];
-        for (long long base = h0; base < h1; base += kThreads) {
-            const long long i = base + tid;
-            int h = 0;
-            bool take = false;
-            if (i < h1) {
-                h = (int)L.hol_list[i];
-                take = sphere_take_hole(C, hol[h]);
-            }
-            const int slot = block_compact(take, nhg, s_wcount, lane, wave, below);
-            if (take && slot < kCapHoles) {
-                if (grouped) s_hgrp[slot] = (unsigned short)h;
-                else s_hcand[slot] = (unsigned short)h;
-            }
-        }
-        if (ng > kCapGroup || nhg > kCapHoles) {  // host retries with smaller tiles
-            if (tid == 0) atomicOr(status, 1);
-            return;
-        }
-        if (tid == 0) {
-            s_meta[0] = tx0; s_meta[1] = ty0; s_meta[2] = tz0;
-            s_meta[3] = tx1 - tx0; s_meta[4] = ty1 - ty0; s_meta[5] = tz1 - tz0;
-            s_meta[6] = ng; s_meta[7] = nhg;
-        }
-        __syncthreads();
-    }
-
     const int S = p.S;
     const float inv_nv3 = 1.0f / (float)(p.nv * p.nv * p.nv);
     // crystal-d sub-point geometry
@@ -392,6 +424,61 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
     const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
 
+    // ---- group cull: the group's voxel-centre sphere over the detector's
+    //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
+    //      then culls these shorter lists (a group of one tile writes its tile
+    //      lists directly).  Ordered compaction keeps every list sorted by index.
+    const bool grouped = (tx1 - tx0) * (ty1 - ty0) * (tz1 - tz0) > 1;
+    {
+        int ng = 0, nhg = 0;
+        float tcx, tcy, tcz, rho;
+        box_sphere(p, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, tcx, tcy, tcz, rho);
+        const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+        const bool gb_ok = group_bundle(p, slb, D, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, pex, pey, hvz, qex, qhy, qez,
+                                        lane, wave, s_tbox);
+        const long long c0 = L.cry_ptr[d], c1 = L.cry_ptr[d + 1];
+        for (long long base = c0; base < c1; base += kThreads) {
+            const long long i = base + tid;
+            int k = 0;
+            bool take = false;
+            if (i < c1) {
+                k = (int)L.cry_list[i];
+                take = sphere_take_crystal(C, cry, k) && (!gb_ok || bundle_take_crystal(cry, k, s_tbox));
+            }
+            const int slot = block_compact(take, ng, s_wcount, lane, wave, below);
+            if (take && slot < kCapGroup) {
+                if (grouped) s_grp[slot] = (unsigned short)k;
+                else s_cand[slot] = (unsigned short)k;
+            }
+        }
+        const long long h0 = L.hol_ptr[d], h1 = L.hol_ptr[d + 1];
+        for (long long base = h0; base < h1; base += kThreads) {
+            const long long i = base + tid;
+            int h = 0;
+            bool take = false;
+            if (i < h1) {
+                h = (int)L.hol_list[i];
+                const Hole H = hol[h];
+                take = sphere_take_hole(C, H) && (!gb_ok || bundle_take_hole(H, s_tbox));
+            }
+            const int slot = block_compact(take, nhg, s_wcount, lane, wave, below);
+            if (take && slot < kCapHoles) {
+                if (grouped) s_hgrp[slot] = (unsigned short)h;
+                else s_hcand[slot] = (unsigned short)h;
+            }
+        }
+        if (ng > kCapGroup || nhg > kCapHoles) {  // host retries with smaller tiles
+            if (tid == 0) atomicOr(status, 1);
+            return;
+        }
+        if (tid == 0) {
+            s_meta[0] = tx0; s_meta[1] = ty0; s_meta[2] = tz0;
+            s_meta[3] = tx1 - tx0; s_meta[4] = ty1 - ty0; s_meta[5] = tz1 - tz0;
+            s_meta[6] = ng; s_meta[7] = nhg;
+        }
+        __syncthreads();
+    }
+
     for (int t = 0; t < uniform(s_meta[3] * s_meta[4] * s_meta[5]); ++t) {
         const int gnx = uniform(s_meta[3]), gny = uniform(s_meta[4]);
         const int tix = uniform(s_meta[0]) + t % gnx, tiy = uniform(s_meta[1]) + (t / gnx) % gny;
@@ -409,6 +496,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             float tcx, tcy, tcz, rho;
             box_sphere(p, vx0, vy0, vz0, vx1, vy1, vz1, tcx, tcy, tcz, rho);
             const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+            const bool tb_ok = group_bundle(p, slb, D, vx0, vy0, vz0, vx1, vy1, vz1, pex, pey, hvz, qex, qhy, qez,
+                                            lane, wave, s_tbox);
             const int ng = uniform(s_meta[6]), nhg = uniform(s_meta[7]);
             for (int base = 0; base < ng; base += kThreads) {
                 const int i = base + tid;
@@ -416,7 +505,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                 bool take = false;
                 if (i < ng) {
                     k = (int)s_grp[i];
-                    take = sphere_take_crystal(C, cry, k);
+                    take = sphere_take_crystal(C, cry, k) && (!tb_ok || bundle_take_crystal(cry, k, s_tbox));
                 }
                 const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
                 if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
@@ -427,7 +516,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                 bool take = false;
                 if (i < nhg) {
                     h = (int)s_hgrp[i];
-                    take = sphere_take_hole(C, hol[h]);
+                    const Hole H = hol[h];
+                    take = sphere_take_hole(C, H) && (!tb_ok || bundle_take_hole(H, s_tbox));
                 }
                 const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
                 if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
@@ -458,34 +548,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
             const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - qhy, Q_yhi = D.cy + qhy;
             const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
-            if (bundle_ok) {
-                // lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face; lanes >= 32 do z.  Each vertex
-                // ray is taken at the slab face clamped to its own y-range [Py, Qy] (a segment
-                // covers no other y): the lateral position a + (b - a) clamp(t, 0, 1) is monotone
-                // in each end-point coordinate, so the 32 vertex rays still bound every sub-ray
-                // segment in the slab.  The 2e-3 mm + 1e-5 relative margin covers the rounding
-                // of the vertex rays and of the sub-point coordinates.
-                const bool zaxis = lane >= 32;
-                const int v = lane & 31;
-                const float Pa = zaxis ? ((v & 1) ? zI + hvz : zI - hvz) : ((v & 1) ? xI + pex : xI - pex);
-                const float Py = (v & 2) ? P_yhi : P_ylo;
-                const float Qa = zaxis ? ((v & 4) ? D.cz + qez : D.cz - qez) : ((v & 4) ? D.cx + qex : D.cx - qex);
-                const float Qy = (v & 8) ? Q_yhi : Q_ylo;
-                const float slope = (Qa - Pa) / (Qy - Py);
-                for (int s = 0; s < p.n_slabs; ++s) {
-                    const Slab SL = slb[s];
-                    const float y = (v & 16) ? SL.y2 : SL.y1;
-                    const float val = Pa + slope * (fminf(fmaxf(y, Py), Qy) - Py);
-                    const float rmn = row_min16(val), rmx = row_max16(val);
-                    const float mnx = fminf(lane_value(rmn, 0), lane_value(rmn, 16));
-                    const float mxx = fmaxf(lane_value(rmx, 0), lane_value(rmx, 16));
-                    const float mnz = fminf(lane_value(rmn, 32), lane_value(rmn, 48));
-                    const float mxz = fmaxf(lane_value(rmx, 32), lane_value(rmx, 48));
-                    const float mgx = 2e-3f + 1e-5f * fmaxf(fabsf(mnx), fabsf(mxx));
-                    const float mgz = 2e-3f + 1e-5f * fmaxf(fabsf(mnz), fabsf(mxz));
-                    if (lane == 0) s_box[wave][s] = make_float4(mnx - mgx, mxx + mgx, mnz - mgz, mxz + mgz);
-                }
-            }
+            if (bundle_ok)
+                bundle_boxes(p, slb, lane, xI, yI, zI, pex, pey, hvz, D.cx, D.cy, D.cz, qex, qhy, qez, s_box[wave]);
             __builtin_amdgcn_wave_barrier();
 
             // -- crystals: bundle overlap, then the exact predicate; survivors are
