@@ -37,6 +37,7 @@ constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kListCap = 192;   // per-wave compacted attenuator boxes (beyond: bitmask path)
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
+constexpr int kSubTab = 256;    // sub-ray index decompositions tabulated per workgroup (S <= kSubTab)
 
 // Lane exchange within each row of 16 lanes (DPP): quad_perm [1,0,3,2],
 // quad_perm [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with
@@ -373,6 +374,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     __shared__ float4 s_tbox[kMaxSlabs];  // bundle boxes of the tile group, then of each tile
     __shared__ int s_wcount[kWaves];
     __shared__ int s_meta[8];  // tile-group state, re-read per tile (keeps it out of scalar registers)
+    __shared__ uint2 s_sub[kSubTab];  // (ny | nx << 16, nz | va << 8 | vb << 16 | vc << 24) of sub-ray s
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -423,6 +425,22 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
     const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
     const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
+
+    // sub-ray s -> (crystal sub-cell, voxel sub-point) indices; the group cull's
+    // barriers order these writes before the passes read them
+    const bool sub_tab = S <= kSubTab;
+    if (sub_tab) {
+        for (int s = tid; s < S; s += kThreads) {
+            int ny, nx, nz, va = 0, vb = 0;
+            int q = divmod(s, p.cy, rcy, ny);
+            q = divmod(q, p.cx, rcx, nx);
+            const int iv = divmod(q, p.cz, rcz, nz);
+            const int t1 = divmod(iv, p.nv, rnv, va);
+            const int vc = divmod(t1, p.nv, rnv, vb);
+            s_sub[s] = make_uint2((unsigned)ny | ((unsigned)nx << 16),
+                                  (unsigned)nz | ((unsigned)va << 8) | ((unsigned)vb << 16) | ((unsigned)vc << 24));
+        }
+    }
 
     // ---- group cull: the group's voxel-centre sphere over the detector's
     //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
@@ -692,15 +710,26 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     const int s = base + lane;
                     if (s >= S) break;
                     // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
-                    int ny, nx, nz;
-                    int q = divmod(s, p.cy, rcy, ny);
-                    q = divmod(q, p.cx, rcx, nx);
-                    const int iv = divmod(q, p.cz, rcz, nz);
+                    int ny, nx, nz, va = 0, vb = 0, vc = 0;
+                    if (sub_tab) {
+                        const uint2 e = s_sub[s];
+                        ny = (int)(e.x & 0xFFFFu);
+                        nx = (int)(e.x >> 16);
+                        nz = (int)(e.y & 0xFFu);
+                        va = (int)((e.y >> 8) & 0xFFu);
+                        vb = (int)((e.y >> 16) & 0xFFu);
+                        vc = (int)(e.y >> 24);
+                    } else {
+                        int q = divmod(s, p.cy, rcy, ny);
+                        q = divmod(q, p.cx, rcx, nx);
+                        const int iv = divmod(q, p.cz, rcz, nz);
+                        if (p.nv > 1) {
+                            const int t1 = divmod(iv, p.nv, rnv, va);
+                            vc = divmod(t1, p.nv, rnv, vb);
+                        }
+                    }
                     float xP = xI, yP = yI, zP = zI;
-                    int va = 0, vb = 0, vc = 0;
                     if (p.nv > 1) {
-                        const int t1 = divmod(iv, p.nv, rnv, va);
-                        vc = divmod(t1, p.nv, rnv, vb);
                         const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
                         const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
                         const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
