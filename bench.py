@@ -47,6 +47,16 @@ def load_flops(key):
         return None
 
 
+def load_profile_figure(key, name="pe_tile_kernel_c2.md"):
+    """A ``key: value`` figure from a committed rocprofv3 summary (None when absent)."""
+    try:
+        text = open(os.path.join(ROOT, "profiles", name)).read()
+    except OSError:
+        return None
+    m = re.search(re.escape(key) + r":\s*([0-9.eE+-]+)", text)
+    return float(m.group(1)) if m else None
+
+
 def load_traffic(name="pe_tile_kernel_c2.md"):
     """HBM bytes per launch (FETCH + WRITE, calibrated) from the committed rocprofv3 summary."""
     path = os.path.join(ROOT, "profiles", name)
@@ -436,7 +446,11 @@ def main():
                 "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
                 "flops_per_element": round(fpe, 1), "kernel": "pe_tile_kernel",
                 "avg_launch_ms": round(avg_kernel_s * 1e3, 3), "rows_per_launch": B,
-                "algorithmic_output_bytes_per_launch": B * V * 4}
+                "algorithmic_output_bytes_per_launch": B * V * 4,
+                "flops_basis": "FP32 operations of the reference algorithm per element (op-counting restatement "
+                               "of photodetectorCudaMe, every crystal its cull keeps); the tracer skips crystals "
+                               "and holes whose chord is provably zero for every sub-ray, so frac can exceed 1",
+                "valu_issue_frac": load_profile_figure("valu_issue_frac")}
 
     if args.calib_copy:
         a = torch.ones(64 * 1024 * 1024, dtype=torch.float32, device=dev)
