@@ -39,28 +39,6 @@ constexpr int kListCap = 192;   // per-wave compacted attenuator boxes (beyond: 
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
 constexpr int kSubTab = 256;    // sub-ray index decompositions tabulated per workgroup (S <= kSubTab)
 
-// Lane exchange within each row of 16 lanes (DPP): quad_perm [1,0,3,2],
-// quad_perm [2,3,0,1], row_half_mirror, row_mirror.  Applied in this order with
-// min / max, every lane of a row ends with the row's minimum / maximum.
-template <int kCtrl>
-__device__ inline float dpp(float v) {
-    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), kCtrl, 0xF, 0xF, false));
-}
-__device__ inline float row_min16(float v) {
-    v = fminf(v, dpp<0xB1>(v));
-    v = fminf(v, dpp<0x4E>(v));
-    v = fminf(v, dpp<0x141>(v));
-    return fminf(v, dpp<0x140>(v));
-}
-__device__ inline float row_max16(float v) {
-    v = fmaxf(v, dpp<0xB1>(v));
-    v = fmaxf(v, dpp<0x4E>(v));
-    v = fmaxf(v, dpp<0x141>(v));
-    return fmaxf(v, dpp<0x140>(v));
-}
-__device__ inline float lane_value(float v, int l) {
-    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), l));
-}
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
     for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
@@ -233,37 +211,37 @@ __device__ inline void box_extent(const PeParams& p, int vx0, int vy0, int vz0, 
 
 // Lateral (x, z) bounds, per y-slab, of every segment from a point of box P
 // (centre pc, half extents ph) to a point of box Q (centre qc, half extents qh),
-// for P entirely before Q in y.  Lane bits: 0 Pa, 1 Py, 2 Qa, 3 Qy, 4 slab face;
-// lanes >= 32 do z.  Each of the 32 vertex rays is taken at the slab face
-// clamped to its own y-range [Py, Qy] (a segment covers no other y): the lateral
-// position a + (b - a) clamp(t, 0, 1) is monotone in each end-point coordinate,
-// so the vertex rays bound every segment in the slab.  The 2e-3 mm + 1e-5
-// relative margin covers the rounding of the vertex rays and of the points.
-// Lane 0 writes box[s] = (x lo, x hi, z lo, z hi); the whole wave calls this.
-// A crystal or hole of slab s outside box[s] has a zero chord on every segment.
+// for P entirely before Q in y.  A segment's lateral position at height y is
+// a + (b - a) t, t = clamp((y - Py) / (Qy - Py), 0, 1), with a, b its lateral end
+// points: non-decreasing in a and b, and t non-increasing in Py and Qy and
+// non-decreasing in y (a segment covers no y outside [Py, Qy]).  So the minimum
+// over the boxes and the slab takes the low lateral ends and, when they rise
+// toward Q, the smallest t (high Py, Qy, lower face) -- else the largest; the
+// maximum likewise with the high ends.  One lane per (slab, axis): lanes 0..15
+// do x, lanes 32..47 z.  The 2e-3 mm + 1e-5 relative margin covers the rounding
+// of these evaluations and of the points themselves.  The whole wave calls this;
+// box[s] = (x lo, x hi, z lo, z hi).  A crystal or hole of slab s outside box[s]
+// has a zero chord on every segment.
 __device__ inline void bundle_boxes(const PeParams& p, const Slab* __restrict__ slb, int lane, float pcx, float pcy,
                                     float pcz, float phx, float phy, float phz, float qcx, float qcy, float qcz,
                                     float qhx, float qhy, float qhz, float4* box) {
     const bool zaxis = lane >= 32;
-    const int v = lane & 31;
-    const float Pa = zaxis ? ((v & 1) ? pcz + phz : pcz - phz) : ((v & 1) ? pcx + phx : pcx - phx);
-    const float Py = (v & 2) ? pcy + phy : pcy - phy;
-    const float Qa = zaxis ? ((v & 4) ? qcz + qhz : qcz - qhz) : ((v & 4) ? qcx + qhx : qcx - qhx);
-    const float Qy = (v & 8) ? qcy + qhy : qcy - qhy;
-    const float slope = (Qa - Pa) / (Qy - Py);
-    for (int s = 0; s < p.n_slabs; ++s) {
-        const Slab SL = slb[s];
-        const float y = (v & 16) ? SL.y2 : SL.y1;
-        const float val = Pa + slope * (fminf(fmaxf(y, Py), Qy) - Py);
-        const float rmn = row_min16(val), rmx = row_max16(val);
-        const float mnx = fminf(lane_value(rmn, 0), lane_value(rmn, 16));
-        const float mxx = fmaxf(lane_value(rmx, 0), lane_value(rmx, 16));
-        const float mnz = fminf(lane_value(rmn, 32), lane_value(rmn, 48));
-        const float mxz = fmaxf(lane_value(rmx, 32), lane_value(rmx, 48));
-        const float mgx = 2e-3f + 1e-5f * fmaxf(fabsf(mnx), fabsf(mxx));
-        const float mgz = 2e-3f + 1e-5f * fmaxf(fabsf(mnz), fabsf(mxz));
-        if (lane == 0) box[s] = make_float4(mnx - mgx, mxx + mgx, mnz - mgz, mxz + mgz);
-    }
+    const int s = lane & 31;
+    if (s >= p.n_slabs) return;
+    const Slab SL = slb[s];
+    const float pc = zaxis ? pcz : pcx, ph = zaxis ? phz : phx;
+    const float qc = zaxis ? qcz : qcx, qh = zaxis ? qhz : qhx;
+    const float Pylo = pcy - phy, Pyhi = pcy + phy, Qylo = qcy - qhy, Qyhi = qcy + qhy;
+    const float Pl = pc - ph, Ql = qc - qh, Ph = pc + ph, Qh = qc + qh;
+    const bool up_l = Ql >= Pl, up_h = Qh >= Ph;
+    const float ay = up_l ? Pyhi : Pylo, aq = up_l ? Qyhi : Qylo, af = up_l ? SL.y1 : SL.y2;
+    const float by = up_h ? Pylo : Pyhi, bq = up_h ? Qylo : Qyhi, bf = up_h ? SL.y2 : SL.y1;
+    const float mn = Pl + ((Ql - Pl) / (aq - ay)) * (fminf(fmaxf(af, ay), aq) - ay);
+    const float mx = Ph + ((Qh - Ph) / (bq - by)) * (fminf(fmaxf(bf, by), bq) - by);
+    const float mg = 2e-3f + 1e-5f * fmaxf(fabsf(mn), fabsf(mx));
+    float* b = reinterpret_cast<float*>(box + s) + (zaxis ? 2 : 0);
+    b[0] = mn - mg;
+    b[1] = mx + mg;
 }
 
 // Crystal k / hole H against the bundle boxes (true: may have a non-zero chord).
