@@ -125,7 +125,7 @@ static int build_lists(PePlanImpl& P) {
     if ((rc = counts.alloc(2 * (size_t)n)) < 0) return rc;
     if ((rc = P.cry_ptr.alloc((size_t)n + 1)) < 0) return rc;
     if ((rc = P.hol_ptr.alloc((size_t)n + 1)) < 0) return rc;
-    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, 0, counts.ptr, P.lists(), nullptr));
+    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, P.slb.ptr, 0, counts.ptr, P.lists(), nullptr));
     std::vector<int> c(2 * (size_t)n);
     HIP_TRY(hipMemcpy(c.data(), counts.ptr, c.size() * sizeof(int), hipMemcpyDeviceToHost));
     std::vector<long long> cp((size_t)n + 1, 0), hp((size_t)n + 1, 0);
@@ -137,7 +137,7 @@ static int build_lists(PePlanImpl& P) {
     HIP_TRY(hipMemcpy(P.hol_ptr.ptr, hp.data(), hp.size() * sizeof(long long), hipMemcpyHostToDevice));
     if ((rc = P.cry_list.alloc((size_t)cp[n])) < 0) return rc;
     if ((rc = P.hol_list.alloc((size_t)hp[n])) < 0) return rc;
-    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, 1, counts.ptr, P.lists(), nullptr));
+    HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, P.slb.ptr, 1, counts.ptr, P.lists(), nullptr));
     HIP_TRY(hipDeviceSynchronize());
     return SM_OK;
 }

@@ -261,6 +261,28 @@ __device__ inline bool bundle_take_hole(const Hole& H, const float4* box) {
     return !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
 }
 
+// Half extents of the sub-point clouds: voxel sub-points around a voxel centre
+// (rotated frame) and the sub-cell centres of crystal D around its centre (world
+// AABB of the rotated cloud).
+struct SubBoxes {
+    float pex, pey, pez;  // voxel sub-points
+    float qex, qhy, qez;  // sub-cell centres of D
+};
+
+__device__ inline SubBoxes sub_boxes(const PeParams& p, const Crystal& D, float cD, float sD) {
+    SubBoxes b;
+    const float hvx = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
+    const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
+    b.pez = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
+    b.pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy;
+    b.pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
+    const float qhx = D.hx * (1.0f - 1.0f / (float)p.cx), qhz = D.hz * (1.0f - 1.0f / (float)p.cz);
+    b.qhy = D.hy * (1.0f - 1.0f / (float)p.cy);
+    b.qex = fabsf(cD) * qhx + fabsf(sD) * qhz;
+    b.qez = fabsf(sD) * qhx + fabsf(cD) * qhz;
+    return b;
+}
+
 // Bundle boxes of every sub-ray from the voxel sub-points of the box [v0, v1) to
 // the sub-cell centres of crystal D (half extents pex/pey/pez, qex/qhy/qez) into
 // box, computed by wave 0; returns whether they apply (P entirely more than 1 mm
@@ -304,9 +326,11 @@ __device__ inline int block_compact(bool take, int& base_count, int* s_wcount, i
 // element's list (and its index order) unchanged.  fill = 0 counts into
 // counts[2 d], counts[2 d + 1]; fill = 1 writes the lists at the offsets.
 __global__ void __launch_bounds__(kThreads) pe_detector_cull_kernel(PeParams p, const Crystal* __restrict__ cry,
-                                                                    const Hole* __restrict__ hol, int fill,
+                                                                    const Hole* __restrict__ hol,
+                                                                    const Slab* __restrict__ slb, int fill,
                                                                     int* __restrict__ counts, PeLists L) {
     __shared__ int s_wcount[kWaves];
+    __shared__ float4 s_box[kMaxSlabs];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
     const int d = blockIdx.x;
@@ -316,17 +340,25 @@ __global__ void __launch_bounds__(kThreads) pe_detector_cull_kernel(PeParams p, 
         float tcx, tcy, tcz, rho;
         box_sphere(p, 0, 0, 0, p.X, p.Y, p.Zslab, tcx, tcy, tcz, rho);
         const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+        const SubBoxes B = sub_boxes(p, D, cosf(D.rot), sinf(D.rot));
+        const bool b_ok = group_bundle(p, slb, D, 0, 0, 0, p.X, p.Y, p.Zslab, B.pex, B.pey, B.pez, B.qex, B.qhy,
+                                       B.qez, lane, wave, s_box);
         unsigned short* cl = fill ? L.cry_list + L.cry_ptr[d] : nullptr;
         unsigned short* hl = fill ? L.hol_list + L.hol_ptr[d] : nullptr;
         for (int base = 0; base < p.n_det; base += kThreads) {
             const int k = base + tid;
-            const bool take = k < p.n_det && k != d && sphere_take_crystal(C, cry, k);
+            const bool take = k < p.n_det && k != d && sphere_take_crystal(C, cry, k) &&
+                              (!b_ok || bundle_take_crystal(cry, k, s_box));
             const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
             if (take && fill) cl[slot] = (unsigned short)k;
         }
         for (int base = 0; base < p.n_holes; base += kThreads) {
             const int h = base + tid;
-            const bool take = h < p.n_holes && sphere_take_hole(C, hol[h]);
+            bool take = false;
+            if (h < p.n_holes) {
+                const Hole H = hol[h];
+                take = sphere_take_hole(C, H) && (!b_ok || bundle_take_hole(H, s_box));
+            }
             const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
             if (take && fill) hl[slot] = (unsigned short)h;
         }
@@ -395,14 +427,9 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
     const float absorb_scale = D.mu_pe / D.mu_t;
-    // world AABB of the sub-cell centres of crystal d (bundle filter end box): the
-    // centres lie within (w/2)(1 - 1/n) of the crystal centre along each crystal axis
-    const float qhx = D.hx * (1.0f - rcx), qhy = D.hy * (1.0f - rcy), qhz = D.hz * (1.0f - rcz);
-    const float qex = fabsf(cD) * qhx + fabsf(sD) * qhz, qez = fabsf(sD) * qhx + fabsf(cD) * qhz;
-    const float hvx = 0.5f * p.wx * (1.0f - 1.0f / (float)p.nv);
-    const float hvy = 0.5f * p.wy * (1.0f - 1.0f / (float)p.nv);
-    const float hvz = 0.5f * p.wz * (1.0f - 1.0f / (float)p.nv);
-    const float pex = fabsf(p.rc) * hvx + fabsf(p.rs) * hvy, pey = fabsf(p.rs) * hvx + fabsf(p.rc) * hvy;
+    // bundle filter end boxes: voxel sub-points, sub-cell centres of crystal d
+    const SubBoxes B = sub_boxes(p, D, cD, sD);
+    const float pex = B.pex, pey = B.pey, hvz = B.pez, qex = B.qex, qhy = B.qhy, qez = B.qez;
 
     // sub-ray s -> (crystal sub-cell, voxel sub-point) indices; the group cull's
     // barriers order these writes before the passes read them
@@ -892,11 +919,11 @@ hipError_t launch_pe(const PeParams& p0, const Crystal* cry, const Hole* hol, co
 
 namespace sysmat {
 
-hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, int fill, int* counts,
-                                const PeLists& L, hipStream_t stream) {
+hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, const Slab* slb, int fill,
+                                int* counts, const PeLists& L, hipStream_t stream) {
     if (p.n_det <= 0) return hipSuccess;
-    hipLaunchKernelGGL(pe_detector_cull_kernel, dim3((unsigned)p.n_det), dim3(kThreads), 0, stream, p, cry, hol, fill,
-                       counts, L);
+    hipLaunchKernelGGL(pe_detector_cull_kernel, dim3((unsigned)p.n_det), dim3(kThreads), 0, stream, p, cry, hol, slb,
+                       fill, counts, L);
     return hipGetLastError();
 }
 

@@ -55,7 +55,7 @@ hipError_t launch_pe(const PeParams& p, const Crystal* cry, const Hole* hol, con
                      const PeLists& L, float* dst, int* status, hipStream_t stream);
 // fill = 0: counts[2 d] / counts[2 d + 1] = list lengths of detector d; fill = 1:
 // write the lists at L's offsets.  One workgroup per detector of the table.
-hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, int fill, int* counts,
-                                const PeLists& L, hipStream_t stream);
+hipError_t launch_detector_cull(const PeParams& p, const Crystal* cry, const Hole* hol, const Slab* slb, int fill,
+                                int* counts, const PeLists& L, hipStream_t stream);
 
 }  // namespace sysmat
