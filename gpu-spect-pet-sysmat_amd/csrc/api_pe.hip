@@ -76,6 +76,10 @@ static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, 
     long long S = (long long)p.nv * p.nv * p.nv * p.cx * p.cy * p.cz;
     if (S > (1 << 24)) return set_error(SM_E_ARG, "too many sub-rays per element");
     p.S = (int)S;
+    p.rnv64 = 1.0 / (double)p.nv;
+    p.rcx64 = 1.0 / (double)p.cx;
+    p.rcy64 = 1.0 / (double)p.cy;
+    p.rcz64 = 1.0 / (double)p.cz;
     p.n_det = T.n_det;
     p.n_layers = T.n_layers;
     p.n_holes = T.n_holes;

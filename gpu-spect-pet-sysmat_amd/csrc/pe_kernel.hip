@@ -45,6 +45,14 @@ __device__ inline double wave_sum64(double v) {
     return v;
 }
 
+// a / b from rb = 1 / b (correctly rounded): q = a rb, then one residual
+// correction with FMA (Markstein's final step), for |a| and b well inside the
+// normal range.
+__device__ inline double div_rcp(double a, double b, double rb) {
+    const double q = a * rb;
+    return fma(fma(-q, b, a), rb, q);
+}
+
 // Wave-uniform copy of an integer (bit-preserving for unsigned).
 __device__ inline int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 __device__ inline unsigned uniform(unsigned v) { return (unsigned)__builtin_amdgcn_readfirstlane((int)v); }
@@ -782,19 +790,20 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                             if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
                         }
                     } else {
-                        // end points in FP64 with the same formulas
+                        // end points in FP64 with the same formulas (quotients by the
+                        // sampling counts through their reciprocals, correctly rounded)
                         double px = xc64, py = yc64, pz = zc64;
                         if (p.nv > 1) {
                             const double fnv = (double)p.nv;
-                            px += (((double)va + 0.5) / fnv - 0.5) * (double)p.wx;
-                            py += (((double)vb + 0.5) / fnv - 0.5) * (double)p.wy;
-                            pz += (((double)vc + 0.5) / fnv - 0.5) * (double)p.wz;
+                            px += (div_rcp((double)va + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wx;
+                            py += (div_rcp((double)vb + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wy;
+                            pz += (div_rcp((double)vc + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wz;
                         }
                         const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
                         const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
-                        const double xs = -w64 / 2.0 + ((double)nx + 0.5) / (double)p.cx * w64;
-                        const double zs = -h64 / 2.0 + ((double)nz + 0.5) / (double)p.cz * h64;
-                        const double ys = -t64 / 2.0 + ((double)ny + 0.5) / (double)p.cy * t64;
+                        const double xs = -w64 / 2.0 + div_rcp((double)nx + 0.5, (double)p.cx, p.rcx64) * w64;
+                        const double zs = -h64 / 2.0 + div_rcp((double)nz + 0.5, (double)p.cz, p.rcz64) * h64;
+                        const double ys = -t64 / 2.0 + div_rcp((double)ny + 0.5, (double)p.cy, p.rcy64) * t64;
                         const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
                         const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
                         const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
@@ -824,7 +833,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                                     }
                                 }
                             }
-                            const double in_plate = (double)PL.thick / R64.ky;
+                            const double in_plate = div_rcp((double)PL.thick, R64.ky, R64.iky);
                             if ((in_plate - holes_len) >= (double)0.00001f)
                                 att64 += (double)PL.mu_t * (in_plate - holes_len);
                         }

@@ -38,6 +38,7 @@ struct PeParams {
     int use_bundle;
     float inv4pi;
     double rc64, rs64;         // image rotation in double (FP64 end points of the collimator path)
+    double rnv64, rcx64, rcy64, rcz64;  // 1 / nv, 1 / cx, 1 / cy, 1 / cz (correctly rounded)
     float fov2c;               // Img[11]
 };
 
