@@ -35,9 +35,35 @@ namespace sysmat {
 
 constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
-constexpr int kListCap = 192;   // per-wave compacted attenuator boxes (beyond: bitmask path)
+constexpr int kListCap = 64;    // per-wave compacted attenuator boxes (beyond: bitmask path)
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
-constexpr int kSubTab = 256;    // sub-ray index decompositions tabulated per workgroup (S <= kSubTab)
+constexpr int kSubTab = 256;    // sub-rays tabulated per workgroup (S <= kSubTab)
+
+// Per-sub-ray constants of a workgroup (one detecting crystal): the index
+// decomposition and the crystal sub-cell geometry (PESysMatGen.cu:606-616, 727-759).
+struct alignas(16) SubCell {
+    unsigned idx0, idx1;                 // ny | nx << 16, nz | va << 8 | vb << 16 | vc << 24
+    float xDs, yDs, zDs;                 // sub-cell centre, crystal frame
+    float xQ, yQ, zQ;                    // the same in the world
+    float ux1, ux2, uy1, uy2, uz1, uz2;  // sub-cell box, crystal frame
+    float pad0, pad1;
+};
+
+__device__ inline void sub_cell_geometry(const Crystal& D, int nx, int ny, int nz, float wD, float tD, float hD,
+                                         float rcx, float rcy, float rcz, float cD, float sD, SubCell& c) {
+    c.xDs = -wD / 2.0f + ((float)nx + 0.5f) * rcx * wD;
+    c.zDs = -hD / 2.0f + ((float)nz + 0.5f) * rcz * hD;
+    c.yDs = -tD / 2.0f + ((float)ny + 0.5f) * rcy * tD;
+    c.xQ = D.cx + (c.xDs * cD - c.zDs * sD);
+    c.zQ = D.cz + (c.xDs * sD + c.zDs * cD);
+    c.yQ = D.cy + c.yDs;
+    c.ux1 = ((float)nx * rcx - 0.5f) * wD;
+    c.ux2 = (((float)nx + 1.0f) * rcx - 0.5f) * wD;
+    c.uy1 = ((float)ny * rcy - 0.5f) * tD;
+    c.uy2 = (((float)ny + 1.0f) * rcy - 0.5f) * tD;
+    c.uz1 = ((float)nz * rcz - 0.5f) * hD;
+    c.uz2 = (((float)nz + 1.0f) * rcz - 0.5f) * hD;
+}
 
 __device__ inline double wave_sum64(double v) {
 #pragma unroll
@@ -392,7 +418,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     __shared__ float4 s_tbox[kMaxSlabs];  // bundle boxes of the tile group, then of each tile
     __shared__ int s_wcount[kWaves];
     __shared__ int s_meta[8];  // tile-group state, re-read per tile (keeps it out of scalar registers)
-    __shared__ uint2 s_sub[kSubTab];  // (ny | nx << 16, nz | va << 8 | vb << 16 | vc << 24) of sub-ray s
+    __shared__ SubCell s_sub[kSubTab];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -439,8 +465,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const SubBoxes B = sub_boxes(p, D, cD, sD);
     const float pex = B.pex, pey = B.pey, hvz = B.pez, qex = B.qex, qhy = B.qhy, qez = B.qez;
 
-    // sub-ray s -> (crystal sub-cell, voxel sub-point) indices; the group cull's
-    // barriers order these writes before the passes read them
+    // sub-ray s -> (crystal sub-cell, voxel sub-point) indices and sub-cell
+    // geometry; the group cull's barriers order these writes before the passes
     const bool sub_tab = S <= kSubTab;
     if (sub_tab) {
         for (int s = tid; s < S; s += kThreads) {
@@ -450,8 +476,12 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             const int iv = divmod(q, p.cz, rcz, nz);
             const int t1 = divmod(iv, p.nv, rnv, va);
             const int vc = divmod(t1, p.nv, rnv, vb);
-            s_sub[s] = make_uint2((unsigned)ny | ((unsigned)nx << 16),
-                                  (unsigned)nz | ((unsigned)va << 8) | ((unsigned)vb << 16) | ((unsigned)vc << 24));
+            SubCell c;
+            c.idx0 = (unsigned)ny | ((unsigned)nx << 16);
+            c.idx1 = (unsigned)nz | ((unsigned)va << 8) | ((unsigned)vb << 16) | ((unsigned)vc << 24);
+            sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, c);
+            c.pad0 = c.pad1 = 0.0f;
+            s_sub[s] = c;
         }
     }
 
@@ -724,14 +754,15 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     if (s >= S) break;
                     // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
                     int ny, nx, nz, va = 0, vb = 0, vc = 0;
+                    SubCell SC;
                     if (sub_tab) {
-                        const uint2 e = s_sub[s];
-                        ny = (int)(e.x & 0xFFFFu);
-                        nx = (int)(e.x >> 16);
-                        nz = (int)(e.y & 0xFFu);
-                        va = (int)((e.y >> 8) & 0xFFu);
-                        vb = (int)((e.y >> 16) & 0xFFu);
-                        vc = (int)(e.y >> 24);
+                        SC = s_sub[s];
+                        ny = (int)(SC.idx0 & 0xFFFFu);
+                        nx = (int)(SC.idx0 >> 16);
+                        nz = (int)(SC.idx1 & 0xFFu);
+                        va = (int)((SC.idx1 >> 8) & 0xFFu);
+                        vb = (int)((SC.idx1 >> 16) & 0xFFu);
+                        vc = (int)(SC.idx1 >> 24);
                     } else {
                         int q = divmod(s, p.cy, rcy, ny);
                         q = divmod(q, p.cx, rcx, nx);
@@ -740,6 +771,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                             const int t1 = divmod(iv, p.nv, rnv, va);
                             vc = divmod(t1, p.nv, rnv, vb);
                         }
+                        sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
                     }
                     float xP = xI, yP = yI, zP = zI;
                     if (p.nv > 1) {
@@ -753,12 +785,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     const float yPs = yP - D.cy;
                     const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
                     // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
-                    const float xDs = -wD / 2.0f + ((float)nx + 0.5f) * rcx * wD;
-                    const float zDs = -hD / 2.0f + ((float)nz + 0.5f) * rcz * hD;
-                    const float yDs = -tD / 2.0f + ((float)ny + 0.5f) * rcy * tD;
-                    const float xQ = D.cx + (xDs * cD - zDs * sD);
-                    const float zQ = D.cz + (xDs * sD + zDs * cD);
-                    const float yQ = D.cy + yDs;
+                    const float xDs = SC.xDs, yDs = SC.yDs, zDs = SC.zDs;
+                    const float xQ = SC.xQ, yQ = SC.yQ, zQ = SC.zQ;
 
                     const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
                     const float dist = R.t;
@@ -870,9 +898,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     }
                     // path inside crystal d before the sub-cell, and absorption in the
                     // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
-                    const float ux1 = ((float)nx * rcx - 0.5f) * wD, ux2 = (((float)nx + 1.0f) * rcx - 0.5f) * wD;
-                    const float uy1 = ((float)ny * rcy - 0.5f) * tD, uy2 = (((float)ny + 1.0f) * rcy - 0.5f) * tD;
-                    const float uz1 = ((float)nz * rcz - 0.5f) * hD, uz2 = (((float)nz + 1.0f) * rcz - 0.5f) * hD;
+                    const float ux1 = SC.ux1, ux2 = SC.ux2, uy1 = SC.uy1, uy2 = SC.uy2, uz1 = SC.uz1, uz2 = SC.uz2;
                     const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
                     const float a1 = open_q(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz);
                     const float a2 = open_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
