@@ -900,11 +900,11 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
                     const float ux1 = SC.ux1, ux2 = SC.ux2, uy1 = SC.uy1, uy2 = SC.uy2, uz1 = SC.uz1, uz2 = SC.uz2;
                     const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
-                    const float a1 = open_q(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz);
-                    const float a2 = open_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
+                    float a1, a2, l_abs;
+                    self_paths(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz, ux1, ux2, uy1, uy2, uz1, uz2, a1, a2,
+                               l_abs);
                     att += (a1 - a2) * D.mu_t;
                     const float eff = expf(-(float)(att64 + (double)att));
-                    const float l_abs = absorb_q(Rq, ux1, ux2, uy1, uy2, uz1, uz2);
                     const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
                     const float contrib = eff * solid * absorb;
                     if (!blocked) acc += (double)contrib;

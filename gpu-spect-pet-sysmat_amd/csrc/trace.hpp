@@ -176,6 +176,43 @@ __device__ inline float absorb_q(const QRay& r, float x1, float x2, float y1, fl
     return (c >= kEps && r.t + sl > kEps && sh < r.t_ext) ? c : 0.0f;
 }
 
+// Closing rules of length_box_ray_inside for a box containing Q, from its slab
+// interval [sl, sh] relative to Q (as in open_q).
+__device__ inline float open_rules(float sl, float sh, float t) {
+    const bool lo_in = (t + sl) > kEps;
+    float v = sh - sl;
+    v = (!lo_in && sh >= 0.0f) ? t : v;
+    v = (!lo_in && sh <= 0.0f) ? t + sh : v;
+    v = (sh >= 0.0f && lo_in) ? -sl : v;
+    return ((sh - sl) < kEps || sl >= 0.0f || t + sh <= kEps) ? 0.0f : v;
+}
+
+// open_q for the crystal box (a1) and the sub-cell box (a2) and absorb_q for the
+// sub-cell box, sharing the slab intervals: the same operations as those three
+// functions, without branches when no component of P -> Q or of the extension is
+// degenerate; otherwise the functions themselves.
+__device__ inline void self_paths(const QRay& r, float x1, float x2, float y1, float y2, float z1, float z2,
+                                  float u1, float u2, float v1, float v2, float w1, float w2, float& a1, float& a2,
+                                  float& l_abs) {
+    if (r.pat | r.pat_e) {
+        a1 = open_q(r, x1, x2, y1, y2, z1, z2);
+        a2 = open_q(r, u1, u2, v1, v2, w1, w2);
+        l_abs = absorb_q(r, u1, u2, v1, v2, w1, w2);
+        return;
+    }
+    float sl, sh, a, b;
+    qslab(r.qy, r.iy, y1, y2, sl, sh);
+    qslab(r.qx, r.ix, x1, x2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b);
+    qslab(r.qz, r.iz, z1, z2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b);
+    a1 = open_rules(sl, sh, r.t);
+    qslab(r.qy, r.iy, v1, v2, sl, sh);
+    qslab(r.qx, r.ix, u1, u2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b);
+    qslab(r.qz, r.iz, w1, w2, a, b); sl = fmaxf(sl, a); sh = fminf(sh, b);
+    a2 = open_rules(sl, sh, r.t);
+    const float c = sh - sl;
+    l_abs = (c >= kEps && r.t + sl > kEps && sh < r.t_ext) ? c : 0.0f;
+}
+
 // Per-sub-ray FP64 data of the collimator path.  The end points themselves are
 // evaluated in FP64: rays nearly parallel to a hole wall amplify end-point
 // rounding (~4e-6 mm in FP32 at 50 mm) by cot(angle) ~ 30-60x into the chord.
