@@ -12,16 +12,20 @@
 //     image slab (pe_detector_cull_kernel);
 //   * one workgroup = 4 wave64s = one detector x a group of 2 x 2 x 2 voxel
 //     tiles (one tile when a launch has few tiles); the workgroup culls the
-//     detector's list against the group's voxel-centre sphere into LDS, then
-//     per tile that list against the tile's sphere (each a proven superset of
-//     every element's cull list);
+//     detector's list for the group into LDS, then per tile that list;
 //   * one wave = one element at a time, its 64 lanes = 64 sub-rays.  Per
-//     element the wave filters the tile candidates lane-parallel -- first the
-//     "bundle" test (every sub-ray of the element bounded laterally in each
-//     y-slab, exact by per-variable monotonicity at the 32 vertex rays: a
-//     crystal outside it has a zero chord for every sub-ray), then the exact
-//     reference cull predicate -- and compacts the survivors' boxes into LDS,
-//     so the attenuator loop is uniform across the wave;
+//     element the wave filters the tile candidates lane-parallel and compacts
+//     the survivors' boxes into LDS, so the attenuator loop is uniform across
+//     the wave;
+//   * every level (image slab, tile group, tile, element) applies the
+//     angular-sphere bound (a proven superset of the reference's per-element
+//     cull) and the "bundle" bound (bundle_boxes: per y-slab lateral limits of
+//     every sub-ray segment; outside them a crystal or hole has a zero chord
+//     for every sub-ray); the element level then applies the reference cull
+//     predicate itself.  Candidate lists stay in index order, so each element
+//     sums its attenuators in the same order at every tile shape;
+//   * per-workgroup LDS table of the sub-ray index decomposition and crystal
+//     sub-cell geometry (S <= 256);
 //   * FP32 tracing; FP64 for the element accumulator and the collimator path
 //     (hole chords and plate length from FP64 end points).
 #include <hip/hip_runtime.h>
