@@ -41,7 +41,8 @@ constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kListCap = 64;    // per-wave compacted attenuator boxes (beyond: bitmask path)
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
-constexpr int kSubTab = 256;    // sub-rays tabulated per workgroup (S <= kSubTab)
+constexpr int kSubTab = 256;    // sub-rays tabulated per workgroup with their geometry (S <= kSubTab)
+constexpr int kIdxTab = 2048;   // sub-ray indices only, in the same LDS (kSubTab < S <= kIdxTab)
 
 // Per-sub-ray constants of a workgroup (one detecting crystal): the index
 // decomposition and the crystal sub-cell geometry (PESysMatGen.cu:606-616, 727-759).
@@ -472,7 +473,11 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     // sub-ray s -> (crystal sub-cell, voxel sub-point) indices and sub-cell
     // geometry; the group cull's barriers order these writes before the passes
     const bool sub_tab = S <= kSubTab;
-    if (sub_tab) {
+    // indices only: nx, ny < 2^16 and nz, va, vb, vc < 2^8 packed in 8 bytes
+    const bool idx_tab = !sub_tab && S <= kIdxTab && p.cx < 65536 && p.cy < 65536 && p.cz < 256 && p.nv < 256;
+    uint2* s_idx = reinterpret_cast<uint2*>(s_sub);
+    static_assert(sizeof(SubCell) * kSubTab >= sizeof(uint2) * kIdxTab, "index table shares the sub-cell table");
+    if (sub_tab || idx_tab) {
         for (int s = tid; s < S; s += kThreads) {
             int ny, nx, nz, va = 0, vb = 0;
             int q = divmod(s, p.cy, rcy, ny);
@@ -483,6 +488,10 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             SubCell c;
             c.idx0 = (unsigned)ny | ((unsigned)nx << 16);
             c.idx1 = (unsigned)nz | ((unsigned)va << 8) | ((unsigned)vb << 16) | ((unsigned)vc << 24);
+            if (idx_tab) {
+                s_idx[s] = make_uint2(c.idx0, c.idx1);
+                continue;
+            }
             sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, c);
             c.pad0 = c.pad1 = 0.0f;
             s_sub[s] = c;
@@ -767,6 +776,15 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                         va = (int)((SC.idx1 >> 8) & 0xFFu);
                         vb = (int)((SC.idx1 >> 16) & 0xFFu);
                         vc = (int)(SC.idx1 >> 24);
+                    } else if (idx_tab) {
+                        const uint2 e = s_idx[s];
+                        ny = (int)(e.x & 0xFFFFu);
+                        nx = (int)(e.x >> 16);
+                        nz = (int)(e.y & 0xFFu);
+                        va = (int)((e.y >> 8) & 0xFFu);
+                        vb = (int)((e.y >> 16) & 0xFFu);
+                        vc = (int)(e.y >> 24);
+                        sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
                     } else {
                         int q = divmod(s, p.cy, rcy, ny);
                         q = divmod(q, p.cx, rcx, nx);
