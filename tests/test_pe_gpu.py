@@ -97,6 +97,24 @@ def test_overflowing_tile_groups_retry_bit_identically(lib):
     assert np.array_equal(a, b) and a.max() > 0
 
 
+def test_tile_groups_bit_identical_to_single_tiles(lib):
+    """16384 crystals, 4^3 tiles: four rows in one launch (4 x 512 workgroups) run 2 x 2 x 2
+    tile groups that share one cull; one row per launch (512 workgroups) runs one tile per
+    workgroup.  Every element's candidate list, in index order, is the same either way."""
+    torch = pytest.importorskip("torch")
+    g = params.config2(seed=0)
+    plan = pe.PePlan(g, sub=(1, 1, 1), tile=(4, 4, 4))
+    r0, n = 12288 + 2080, 4
+    a = torch.empty((n, g.n_vox), dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    plan.launch(r0, n, a)
+    for i in range(n):
+        plan.launch(r0 + i, 1, b[i:])
+    plan.sync()
+    assert torch.equal(a, b) and float(a.max()) > 0.0
+    plan.close()
+
+
 def test_bundle_filter_changes_nothing(lib):
     g = params.tiny(seed=5, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3, n_plates=2)
     a = pe_compute(g, bundle_filter=True)
