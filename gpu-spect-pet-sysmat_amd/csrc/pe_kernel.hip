@@ -466,6 +466,20 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
     const float absorb_scale = D.mu_pe / D.mu_t;
+    // sum of mu * thickness over the plates (FP64 and FP32) for the plate path of
+    // elements without candidate holes; plates under 1e-5 mm are handled per sub-ray
+    double plate_mt = 0.0;
+    float plate_mtf = 0.0f;
+    bool thin_plates = false;
+    for (int m = 0; m < p.n_layers; ++m) {
+        const Plate PL = pla[m];
+        if (PL.thick >= 0.00001f) {
+            plate_mt += (double)PL.mu_t * (double)PL.thick;
+            plate_mtf += PL.mu_t * PL.thick;
+        } else {
+            thin_plates = true;
+        }
+    }
     // bundle filter end boxes: voxel sub-points, sub-cell centres of crystal d
     const SubBoxes B = sub_boxes(p, D, cD, sD);
     const float pex = B.pex, pey = B.pey, hvz = B.pez, qex = B.qex, qhy = B.qhy, qez = B.qez;
@@ -833,11 +847,24 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     double att64 = 0.0;
                     // collimator layers: plate length minus hole chords (PESysMatGen.cu:661-685)
                     if (layer_holes == 0) {
-                        const float icos = dist * frcp(R.dy);
-                        for (int m = 0; m < p.n_layers; ++m) {
-                            const Plate PL = pla[m];
-                            const float in_plate = PL.thick * icos;
-                            if (in_plate >= 0.00001f) att += PL.mu_t * in_plate;
+                        // plate path thick * |PQ| / dy = thick (1 + q), q = r^2 / (1 + sqrt(1 + r^2)),
+                        // r = lateral / dy: mu * thick summed in FP64 and only the small q term
+                        // rounded in FP32, so the exponent of strongly attenuated elements (tens
+                        // of mean free paths) keeps ~1e-7 absolute accuracy.  dy <= 0 gives a
+                        // negative length, which the reference does not add.
+                        if (R.dy > 0.0f) {
+                            const float idy = frcp(R.dy);
+                            const float r2 = (R.dx * R.dx + R.dz * R.dz) * (idy * idy);
+                            const float qv = r2 * frcp(1.0f + fsqrt(1.0f + r2));
+                            att64 += plate_mt + (double)(plate_mtf * qv);
+                            if (thin_plates) {  // plates under 1e-5 mm count only along very oblique paths
+                                for (int m = 0; m < p.n_layers; ++m) {
+                                    const Plate PL = pla[m];
+                                    if (PL.thick < 0.00001f && PL.thick * (1.0f + qv) >= 0.00001f)
+                                        att64 += (double)PL.mu_t * (double)PL.thick +
+                                                 (double)((PL.mu_t * PL.thick) * qv);
+                                }
+                            }
                         }
                     } else {
                         // end points in FP64 with the same formulas (quotients by the

@@ -122,6 +122,19 @@ def test_bundle_filter_changes_nothing(lib):
     assert_parity(a, b, rel=5e-6, what="filter on/off")
 
 
+def test_bundle_filters_change_nothing_on_16384_crystals(lib):
+    """The bundle bounds (per detector, tile group, tile and element) only drop crystals and
+    holes with a zero chord on every sub-ray; back-layer rows of the 16384-crystal geometry,
+    three crystal layers in front, with and without them.  Elements whose every candidate hole
+    drops out take the FP32 plate path instead of the FP64 one, hence the tolerance."""
+    g = params.config2(seed=0)
+    rows = (12288 + 2080, 12288 + 2084)
+    a = pe_compute(g, sub=(2, 2, 2), det_range=rows, tile=(4, 4, 4), bundle_filter=True)
+    b = pe_compute(g, sub=(2, 2, 2), det_range=rows, tile=(4, 4, 4), bundle_filter=False)
+    assert a.max() > 0
+    assert_parity(a, b, rel=5e-6, what="c2 filter on/off")
+
+
 def test_dropin_slabs_match_full_volume(lib):
     """PESysMatGen called per Z-slab with the in-band Img[2]/[30]/[31] fields
     (PEGen_CircularHole.cpp:165-216) reproduces the whole-volume result."""
