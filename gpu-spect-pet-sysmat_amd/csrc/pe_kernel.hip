@@ -136,17 +136,17 @@ __device__ inline int cull_fast(float ex, float ey, float ez, float r, float ux,
     return 0;
 }
 
-// Chord of the sub-ray through crystal k (reference order: PESysMatGen.cu:687-725).
-__device__ inline float crystal_chord(const WRay& R, const Crystal* __restrict__ cry, int k, float x1, float x2,
-                                      float y1, float y2, float z1, float z2, bool rotated) {
-    if (!rotated && R.pat == 0) return chord_world(R, x1, x2, y1, y2, z1, z2);
+// Chord of the sub-ray P -> Q through crystal k (reference order: PESysMatGen.cu:687-725).
+__device__ inline float crystal_chord(const WRay& R, float qx, float qy, float qz, const Crystal* __restrict__ cry,
+                                      int k, float x1, float x2, float y1, float y2, float z1, float z2,
+                                      bool rotated) {
+    if (!rotated && R.pat == 0) return chord_world_q(R, qx, qy, qz, x1, x2, y1, y2, z1, z2);
     const Crystal K = cry[k];
     if (!rotated) {  // along-axis branches, compared in the crystal frame
         const float px = R.px - K.cx, py = R.py - K.cy, pz = R.pz - K.cz;
         return chord_special(px, py, pz, px + R.dx, py + R.dy, pz + R.dz, R.ix, R.iy, R.iz, R.t, R.pat, -K.hx, -K.hy,
                              -K.hz, K.hx, K.hy, K.hz);
     }
-    const float qx = R.px + R.dx, qy = R.py + R.dy, qz = R.pz + R.dz;
     const float xPk = (R.px - K.cx) * K.cr - (R.pz - K.cz) * K.sr, yPk = R.py - K.cy;
     const float zPk = (R.px - K.cx) * K.sr + (R.pz - K.cz) * K.cr;
     const float xQk = (qx - K.cx) * K.cr - (qz - K.cz) * K.sr, yQk = qy - K.cy;
@@ -155,7 +155,7 @@ __device__ inline float crystal_chord(const WRay& R, const Crystal* __restrict__
     if (Rk.pat != 0)
         return chord_special(xPk, yPk, zPk, xQk, yQk, zQk, Rk.ix, Rk.iy, Rk.iz, Rk.t, Rk.pat, -K.hx, -K.hy, -K.hz,
                              K.hx, K.hy, K.hz);
-    return chord_world(Rk, -K.hx, K.hx, -K.hy, K.hy, -K.hz, K.hz);
+    return chord_world_q(Rk, xQk, yQk, zQk, -K.hx, K.hx, -K.hy, K.hy, -K.hz, K.hz);
 }
 
 // Superset cull of a sphere of voxel centres (centre t, radius rho) against
@@ -918,19 +918,20 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     // other crystals on the way (PESysMatGen.cu:687-725)
                     if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
                         // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
-                        // wave; unrolled so the LDS reads of several boxes are in flight together
+                        // wave; parameters measured from Q (the attenuators lie near the detecting
+                        // crystal); unrolled so the LDS reads of several boxes are in flight together
 #pragma unroll 4
                         for (int j = 0; j < n_att; ++j) {
                             const Att A = s_att[wave][j];
-                            att += chord_world(R, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
+                            att += chord_world_q(R, xQ, yQ, zQ, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
                         }
                     } else if (att_fast) {
                         for (int j = 0; j < n_att; ++j) {
                             const Att A = s_att[wave][j];
                             const int k = uniform(A.k);
                             const bool rotated = k < 0;
-                            att += crystal_chord(R, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2,
-                                                 rotated) * A.mu;
+                            att += crystal_chord(R, xQ, yQ, zQ, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2,
+                                                 A.z1, A.z2, rotated) * A.mu;
                         }
                     } else {
                         for (int w = 0; w < ncw; ++w) {
@@ -940,8 +941,9 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                                 msk &= msk - 1;
                                 const int k = uniform((int)s_cand[w * 32 + b]);
                                 const Crystal K = cry[k];
-                                att += crystal_chord(R, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy, K.cy + K.hy,
-                                                     K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) * K.mu_t;
+                                att += crystal_chord(R, xQ, yQ, zQ, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy,
+                                                     K.cy + K.hy, K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) *
+                                       K.mu_t;
                             }
                         }
                     }
@@ -953,7 +955,13 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     self_paths(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz, ux1, ux2, uy1, uy2, uz1, uz2, a1, a2,
                                l_abs);
                     att += (a1 - a2) * D.mu_t;
-                    const float eff = expf(-(float)(att64 + (double)att));
+                    // exp(-a) = 2^-(a log2 e): the integer part of the exponent applied exactly
+                    // (ldexp), the fraction in FP32, so the relative error stays ~2 ulp for any
+                    // attenuation (rounding a itself to FP32 would cost |a| * 6e-8)
+                    const double y2 = (att64 + (double)att) * 1.4426950408889634;
+                    const double yi = floor(y2);
+                    const float eff =
+                        (y2 < 160.0) ? ldexpf(__builtin_amdgcn_exp2f(-(float)(y2 - yi)), -(int)yi) : 0.0f;
                     const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
                     const float contrib = eff * solid * absorb;
                     if (!blocked) acc += (double)contrib;

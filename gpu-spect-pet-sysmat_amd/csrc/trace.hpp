@@ -92,6 +92,23 @@ __device__ inline float chord_world(const WRay& r, float x1, float x2, float y1,
     return (c >= kEps && lo > kEps && hi < r.t) ? c : 0.0f;
 }
 
+// chord_world with the slab parameters measured from the far end point Q
+// instead of P (s = t - |Q - P|): the same chord and conditions (s_min > eps - t,
+// s_max < 0), but the rounding of the parameters scales with the box's distance
+// from Q -- a few mm for the crystals in front of the detecting one -- rather
+// than with its distance from the voxel.
+__device__ inline float chord_world_q(const WRay& r, float qx, float qy, float qz, float x1, float x2, float y1,
+                                      float y2, float z1, float z2) {
+    float a = (x1 - qx) * r.ix, b = (x2 - qx) * r.ix;
+    float lo = fminf(a, b), hi = fmaxf(a, b);
+    a = (y1 - qy) * r.iy; b = (y2 - qy) * r.iy;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    a = (z1 - qz) * r.iz; b = (z2 - qz) * r.iz;
+    lo = fmaxf(lo, fminf(a, b)); hi = fminf(hi, fmaxf(a, b));
+    const float c = hi - lo;
+    return (c >= kEps && lo > kEps - r.t && hi < 0.0f) ? c : 0.0f;
+}
+
 // Sub-ray in the detecting crystal's frame, parameterised from Q (inside the
 // crystal): s = (param from P) - |Q - P|.
 struct QRay {

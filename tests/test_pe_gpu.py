@@ -65,6 +65,18 @@ def test_config1_config_sampling(lib):
     assert_parity(gpu, oracle.pe(g, nv=4, sub=(2, 2, 2), det_range=rows, precision="f64"), what="c1 cfg f64")
 
 
+@pytest.mark.parametrize("row,z,nv", [(2080, 20, 1), (12288 + 2080, 37, 1), (12288 + 2081, 31, 3)])
+def test_config2_row_planes_against_fp64_oracle(lib, row, z, nv):
+    """16384 crystals in 4 layers, two plates with 8 x 8 holes: one front- or back-layer row on
+    one z plane of the 64^3 image (every filter level active on the real geometry) against the
+    FP64 oracle; significance is relative to the plane's own maximum."""
+    g = params.config2(seed=0)
+    gpu = pe_compute(g, voxel_sub=nv, sub=(2, 2, 2), det_range=(row, row + 1), z_range=(z, z + 1))
+    ref = oracle.pe(g, nv=nv, sub=(2, 2, 2), det_range=(row, row + 1), precision="f64", z0=z, slab_z=1)
+    assert gpu.max() > 0
+    assert_parity(gpu, ref, what=f"c2 row {row} z {z} nv {nv}")
+
+
 def test_rotated_image_and_crystals(lib):
     g = params.tiny(seed=3, n_det_side=3, n_layers=1, image=(4, 3, 2), holes_side=2, rot_angle=0.35)
     g.img[8:11] = (0.7, -0.4, 0.3)
