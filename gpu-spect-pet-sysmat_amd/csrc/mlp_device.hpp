@@ -121,4 +121,108 @@ __device__ __forceinline__ void mlp_wave_forward(const uint8_t* __restrict__ P, 
     }
 }
 
+// Block-cooperative forward for a 256-thread workgroup (4 waves x 32 elements):
+// each 16-tile weight slab (output tile pair kbo of a layer: tiles 16 kbo ..
+// 16 kbo + 15, contiguous in the packed layout) is staged once into LDS and
+// read by all four waves, so the L2 reads per element drop fourfold.  The MFMAs
+// and operands are those of mlp_wave_forward, so the outputs are identical.
+// Every thread of the workgroup must call it (barriers); s_w holds 16 KiB.
+__device__ __forceinline__ void stage_slab(const uint8_t* __restrict__ base, long long tile0, uint4* s_w) {
+    __syncthreads();  // the previous slab has been read by every wave
+    const uint4* src = reinterpret_cast<const uint4*>(base + tile0 * 1024);
+#pragma unroll
+    for (int i = 0; i < 4; ++i) s_w[threadIdx.x + 256 * i] = src[threadIdx.x + 256 * i];
+    __syncthreads();
+}
+
+__device__ __forceinline__ bf16x8 lds_tile(const uint4* s_w, int t, int lane) {
+    return __builtin_bit_cast(bf16x8, s_w[t * 64 + lane]);
+}
+
+__device__ __forceinline__ void hidden_layer_block(const uint8_t* __restrict__ W, const float* __restrict__ bias,
+                                                   const bf16x8 (&in)[8][2], bf16x8 (&out)[8][2], int lane,
+                                                   uint4* s_w) {
+    const int g = lane >> 4;
+#pragma unroll
+    for (int kbo = 0; kbo < 8; ++kbo) {
+        stage_slab(W, 16 * kbo, s_w);
+        f32x4 acc[2][2];
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) acc[mm][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kbi = 0; kbi < 8; ++kbi) {
+            const bf16x8 a0 = lds_tile(s_w, kbi, lane);
+            const bf16x8 a1 = lds_tile(s_w, 8 + kbi, lane);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                acc[0][t] = mfma16(a0, in[kbi][t], acc[0][t]);
+                acc[1][t] = mfma16(a1, in[kbi][t], acc[1][t]);
+            }
+        }
+        act_pack(acc, bias, kbo, g, out[kbo]);
+    }
+}
+
+__device__ __forceinline__ void mlp_block_forward(const uint8_t* __restrict__ P, const bf16x8 (&x)[2],
+                                                  float (&y)[2], uint4* s_w) {
+    const int lane = threadIdx.x & 63, g = lane >> 4;
+    const float* bias = reinterpret_cast<const float*>(P + kPkBias);
+    const float* w5 = reinterpret_cast<const float*>(P + kPkW5);
+    bf16x8 A[8][2], B[8][2];
+    stage_slab(P + kPkL1, 0, s_w);
+#pragma unroll
+    for (int kb = 0; kb < 8; ++kb) {
+        const bf16x8 a0 = lds_tile(s_w, 2 * kb, lane), a1 = lds_tile(s_w, 2 * kb + 1, lane);
+        f32x4 acc[2][2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+            acc[0][t] = mfma16(a0, x[t], f32x4{0.f, 0.f, 0.f, 0.f});
+            acc[1][t] = mfma16(a1, x[t], f32x4{0.f, 0.f, 0.f, 0.f});
+        }
+        act_pack(acc, bias, kb, g, A[kb]);
+    }
+    hidden_layer_block(P + kPkL2, bias + 256, A, B, lane, s_w);
+    hidden_layer_block(P + kPkL3, bias + 512, B, A, lane, s_w);
+    float yp[2] = {0.f, 0.f};
+    const float* b4 = bias + 768;
+#pragma unroll
+    for (int kbo = 0; kbo < 8; ++kbo) {
+        stage_slab(P + kPkL4, 16 * kbo, s_w);
+        f32x4 acc[2][2];
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+            for (int t = 0; t < 2; ++t) acc[mm][t] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int kbi = 0; kbi < 8; ++kbi) {
+            const bf16x8 a0 = lds_tile(s_w, kbi, lane);
+            const bf16x8 a1 = lds_tile(s_w, 8 + kbi, lane);
+#pragma unroll
+            for (int t = 0; t < 2; ++t) {
+                acc[0][t] = mfma16(a0, A[kbi][t], acc[0][t]);
+                acc[1][t] = mfma16(a1, A[kbi][t], acc[1][t]);
+            }
+        }
+#pragma unroll
+        for (int mm = 0; mm < 2; ++mm)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int n = 32 * kbo + 16 * mm + 4 * g + i;
+                const float wn = w5[n], bn = b4[n];
+#pragma unroll
+                for (int t = 0; t < 2; ++t) yp[t] += wn * silu_f(acc[mm][t][i] + bn);
+            }
+    }
+    const float b5 = *reinterpret_cast<const float*>(P + kPkB5);
+#pragma unroll
+    for (int t = 0; t < 2; ++t) {
+        float v = yp[t];
+        v += __shfl_xor(v, 16, 64);
+        v += __shfl_xor(v, 32, 64);
+        y[t] = v + b5;
+    }
+}
+
 }  // namespace sysmat

@@ -80,8 +80,10 @@ __device__ inline long long sg_voxel(const SgParams& q, long long ch, int i, boo
 
 // One wave = half a 4 x 4 x 4 voxel brick (32 voxels) of one detector row.
 // Features go through LDS into the MFMA B-operand layout; every member runs
-// register-resident.  Disagreeing members flag the brick (tile) for the exact
-// kernel, which prunes scatter sites per brick.
+// with register-resident activations and weight slabs shared by the
+// workgroup's four waves through LDS (every wave runs every member).
+// Disagreeing members flag the brick (tile) for the exact kernel, which prunes
+// scatter sites per brick.
 __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
                                                               const ScPair* __restrict__ keep,
                                                               const int* __restrict__ keep_cnt,
@@ -89,6 +91,7 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
                                                               const float* __restrict__ pe, float* __restrict__ dst,
                                                               uint8_t* __restrict__ flags) {
     __shared__ float s_f[4][32][kSgFeatures + 1];
+    __shared__ uint4 s_w[16 * 64];  // one weight slab, shared by the four waves
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j16 = lane & 15;
     const long long chunks = 2 * q.n_tiles;
     const long long gw = (long long)blockIdx.x * 4 + w;
@@ -123,7 +126,7 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
     float far_sum[2] = {0.f, 0.f}, far_min[2] = {3.0e38f, 3.0e38f}, far_max[2] = {0.f, 0.f};
     for (int m = 0; m < q.n_members; ++m) {
         float y[2];
-        mlp_wave_forward(packed + (long long)m * kMlpPacked, xf, y);
+        mlp_block_forward(packed + (long long)m * kMlpPacked, xf, y, s_w);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const float fr = fmaxf(__expf(y[t] * q.sig_y + q.mu_y) - q.phi, 0.0f);
