@@ -634,7 +634,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
 
             // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
-            const float P_ylo = yI - pey, P_yhi = yI + pey, Q_ylo = D.cy - qhy, Q_yhi = D.cy + qhy;
+            const float P_yhi = yI + pey, Q_ylo = D.cy - qhy;
             const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
             if (bundle_ok)
                 bundle_boxes(p, slb, lane, xI, yI, zI, pex, pey, hvz, D.cx, D.cy, D.cz, qex, qhy, qez, s_box[wave]);
