@@ -33,7 +33,7 @@ from typing import List, Optional
 
 import numpy as np
 
-from .params import DET_STRIDE, HOLE_BASE, HOLE_STRIDE, MAX_LAYERS, Geometry, count
+from .params import DET_STRIDE, HOLE_BASE, HOLE_STRIDE, MAX_LAYERS, Geometry
 
 REFERENCE_MAX_CRYSTALS = (80000 - 1) // DET_STRIDE
 
