@@ -466,6 +466,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
     const float absorb_scale = D.mu_pe / D.mu_t;
+    const bool axis_d = D.rot == 0.0f;  // crystal frame = translated world frame
     // sum of mu * thickness over the plates (FP64 and FP32) for the plate path of
     // elements without candidate holes; plates under 1e-5 mm are handled per sub-ray
     double plate_mt = 0.0;
@@ -950,7 +951,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     // path inside crystal d before the sub-cell, and absorption in the
                     // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
                     const float ux1 = SC.ux1, ux2 = SC.ux2, uy1 = SC.uy1, uy2 = SC.uy2, uz1 = SC.uz1, uz2 = SC.uz2;
-                    const QRay Rq = make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
+                    const QRay Rq = axis_d ? make_qray_axis(R, xPs, yPs, zPs, xDs, yDs, zDs)
+                                           : make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
                     float a1, a2, l_abs;
                     self_paths(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz, ux1, ux2, uy1, uy2, uz1, uz2, a1, a2,
                                l_abs);

@@ -139,6 +139,25 @@ __device__ inline QRay make_qray(float px, float py, float pz, float qx, float q
     return r;
 }
 
+// make_qray for a crystal whose frame is the world translated (rotation 0): the
+// direction, its length and reciprocals are those of the world ray w (the same
+// difference vector, rounded once more or less), so only the extension is new.
+__device__ inline QRay make_qray_axis(const WRay& w, float px, float py, float pz, float qx, float qy, float qz) {
+    QRay r;
+    r.px = px; r.py = py; r.pz = pz;
+    r.qx = qx; r.qy = qy; r.qz = qz;
+    r.t = w.t;
+    r.pat = w.pat;
+    r.ix = w.ix;
+    r.iy = w.iy;
+    r.iz = w.iz;
+    const float k = 1000.0f * frcp(w.t);
+    const float ex = (qx + k * w.dx) - px, ey = (qy + k * w.dy) - py, ez = (qz + k * w.dz) - pz;
+    r.pat_e = (fabsf(ex) < kEps ? 1 : 0) | (fabsf(ey) < kEps ? 2 : 0) | (fabsf(ez) < kEps ? 4 : 0);
+    r.t_ext = k * r.t;
+    return r;
+}
+
 // Slab interval (relative to Q) of the box on one axis.
 __device__ inline void qslab(float q, float inv, float b1, float b2, float& lo, float& hi) {
     const float a = (b1 - q) * inv, b = (b2 - q) * inv;
