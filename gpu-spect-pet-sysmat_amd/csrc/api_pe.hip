@@ -307,7 +307,7 @@ int sm_pe_compute_device(const float* col, const float* det, const float* img, c
 
 int sm_pe_compute(const float* col, const float* det, const float* img, const SmPeOptions* o, float* dst) {
     clear_error();
-    if (!dst || !o || !img) return set_error(SM_E_ARG, "null argument");
+    if (!col || !det || !img || !o || !dst) return set_error(SM_E_ARG, "null argument");
     int rc = use_device(o->device);
     if (rc < 0) return rc;
     const int X = (int)std::floor(img[0] + 0.001f), Y = (int)std::floor(img[1] + 0.001f);
@@ -351,7 +351,15 @@ int sm_PESysMatGen(const float* col, const float* det, const float* img, float* 
     int rc = sm_pe_plan_create(col, det, img2, &o, &plan);
     if (rc < 0) return rc;
     auto* P = reinterpret_cast<PePlanImpl*>(plan);
-    P->base.Zfull = fullZ;
+    if (P->base.Zfull != fullZ) {
+        // Img[30] unset or short of the slab: the candidate lists follow the
+        // voxel positions, so cull them again for the reference's positions
+        P->base.Zfull = fullZ;
+        if ((rc = build_lists(*P)) < 0) {
+            sm_pe_plan_destroy(plan);
+            return rc;
+        }
+    }
     const size_t n = (size_t)P->base.det_count * X * Y * slabZ;
     DeviceBuf<float> out;
     if ((rc = out.alloc(n)) < 0) {

@@ -658,11 +658,17 @@ int sm_scatter_plan_destroy(SmScatterPlan* plan) {
 int sm_scatter_crystal(const float* det, const float* img, const float* phy, const float* pe,
                        const unsigned* crystal_words_host, int rotation, int det_begin, int det_count, float* dst,
                        int device) {
+    clear_error();
+    if (!pe || !dst) return set_error(SM_E_ARG, "null argument");
     SmScatterPlan* plan = nullptr;
     int rc = sm_scatter_plan_create(nullptr, det, img, phy, rotation, 0, crystal_words_host, nullptr, device, &plan);
     if (rc < 0) return rc;
     const long long V = plan->G.base.V, N = plan->G.T.n_det;
     if (det_count <= 0) det_count = (int)N - det_begin;
+    if (det_begin < 0 || det_count <= 0 || det_begin + (long long)det_count > N) {
+        sm_scatter_plan_destroy(plan);
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    }
     DeviceBuf<float> dpe, dout;
     if ((rc = dpe.alloc((size_t)(N * V))) >= 0 && (rc = dout.alloc((size_t)det_count * V)) >= 0) {
         hipError_t e = hipMemcpy(dpe.ptr, pe, sizeof(float) * N * V, hipMemcpyHostToDevice);
@@ -680,13 +686,17 @@ int sm_scatter_crystal(const float* det, const float* img, const float* phy, con
 int sm_scatter_collimator(const float* col, const float* det, const float* img, const float* phy,
                           const unsigned* collimator_words_host, int rotation, int variant, int det_begin,
                           int det_count, float* dst, int device) {
+    clear_error();
+    if (!dst) return set_error(SM_E_ARG, "null argument");
     SmScatterPlan* plan = nullptr;
-    float* none = nullptr;
-    (void)none;
     int rc = sm_scatter_plan_create(col, det, img, phy, rotation, variant, nullptr, collimator_words_host, device, &plan);
     if (rc < 0) return rc;
     const long long V = plan->G.base.V, N = plan->G.T.n_det;
     if (det_count <= 0) det_count = (int)N - det_begin;
+    if (det_begin < 0 || det_count <= 0 || det_begin + (long long)det_count > N) {
+        sm_scatter_plan_destroy(plan);
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    }
     DeviceBuf<float> dout;
     if ((rc = dout.alloc((size_t)det_count * V)) >= 0) {
         rc = sm_scatter_plan_collimator(plan, det_begin, det_count, dout.ptr, V, nullptr);
@@ -711,6 +721,10 @@ int sm_scatter_cells(const float* col, const float* det, const float* img, const
     if (rc < 0) return rc;
     const long long V = plan->G.base.V, N = plan->G.T.n_det;
     if (det_count <= 0) det_count = (int)N - det_begin;
+    if (det_begin < 0 || det_count <= 0 || det_begin + (long long)det_count > N) {
+        sm_scatter_plan_destroy(plan);
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    }
     DeviceBuf<float> dout;
     if ((rc = dout.alloc((size_t)det_count * V)) >= 0) {
         rc = sm_scatter_plan_cells(plan, cell_size, keep_dist, det_begin, det_count, dout.ptr, V, 0, nullptr);

@@ -196,8 +196,10 @@ def _opts(nv, sub, det_range, n_det, threads, flags=0, jitter=0.0, jseed=0) -> P
 
 def pe(geo, nv: int = 1, sub: Tuple[int, int, int] = (8, 16, 8), det_range=None, precision: str = "f32",
        z0: int = 0, slab_z: Optional[int] = None, rot: int = 0, threads: int = 0, flags: int = 0,
-       jitter: float = 0.0, jseed: int = 0) -> np.ndarray:
+       jitter: float = 0.0, jseed: int = 0, full_z: Optional[int] = None) -> np.ndarray:
     """PE sysmat rows [det_range) x slab voxels, det-major, float32.
+
+    full_z overrides the in-band full Z (img[30]) that places the voxels.
 
     precision: "f32" (reference restatement), "f64" (accuracy oracle) or "mixed"
     (FP32 tracing with an FP64 element accumulator, an FP64 collimator path fed
@@ -206,6 +208,8 @@ def pe(geo, nv: int = 1, sub: Tuple[int, int, int] = (8, 16, 8), det_range=None,
     if precision == "mixed":
         precision, flags = "f32", 39
     img = slab_image(geo.img, z0, slab_z, rot)
+    if full_z is not None:
+        img[30] = full_z
     X, Y = int(np.floor(img[0])), int(np.floor(img[1]))
     bins = X * Y * int(np.floor(img[2]))
     o = _opts(nv, sub, det_range, geo.n_det, threads, flags, jitter, jseed)

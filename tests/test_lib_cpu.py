@@ -36,6 +36,20 @@ def test_version_and_errors_without_gpu():
     assert L.sm_device_count() >= 0
 
 
+def test_null_arguments_are_rejected_before_any_device_call():
+    import ctypes
+    L = _lib.load()
+    o = _lib.SmPeOptions()
+    assert L.sm_pe_default_options(ctypes.byref(o)) == 0
+    buf = np.zeros(256, np.float32)
+    p = buf.ctypes.data
+    assert L.sm_pe_compute(p, None, p, ctypes.byref(o), p) == _lib.SM_E_ARG
+    assert L.sm_scatter_crystal(p, p, p, None, None, 0, 0, 0, p, 0) == _lib.SM_E_ARG
+    assert L.sm_scatter_collimator(p, p, p, p, None, 0, 0, 0, 0, None, 0) == _lib.SM_E_ARG
+    assert L.sm_scatter_cells(None, p, p, p, 0, 0.5, 1.0, 0, 0, p, 0) == _lib.SM_E_ARG
+    assert b"null" in L.sm_last_error()
+
+
 def test_params_roundtrip(tmp_path):
     g = params.config2(seed=3)
     g.write(str(tmp_path))

@@ -168,6 +168,23 @@ def test_dropin_slabs_match_full_volume(lib):
     assert np.array_equal(got, full)
 
 
+def test_dropin_unset_full_z_places_voxels_like_the_reference(lib):
+    """Img[30] = 0 (unset): the reference places plane z0 + iz at (z0 + iz + 0.5) w + shift
+    (PESysMatGen.cu:438-444 with Img[30] = 0), and so must the candidate lists."""
+    g = params.tiny(seed=6, n_det_side=3, n_layers=1, image=(4, 4, 6), holes_side=2)
+    X, Y, Z = g.shape
+    N = g.n_det
+    img = np.zeros(100, np.float32)
+    img[:len(g.img)] = g.img
+    z0, sz = 2, 3
+    img[2], img[30], img[31] = sz, 0, z0
+    slab = np.zeros(N * X * Y * sz, np.float32)
+    assert PESysMatGen(g.col, g.det, img, slab, 0) == X * Y * sz
+    ref = oracle.pe(g, precision="f64", z0=z0, slab_z=sz, full_z=0)
+    assert slab.max() > 0
+    assert_parity(slab.reshape(N, -1), ref, what="unset full Z")
+
+
 def test_zero_mu_detector_rows_are_zero(lib):
     g = params.tiny(seed=7, n_det_side=2, n_layers=1, image=(3, 3, 3), holes_side=2)
     g.crystals[1, 6] = 0.005
