@@ -185,6 +185,16 @@ def test_dropin_unset_full_z_places_voxels_like_the_reference(lib):
     assert_parity(slab.reshape(N, -1), ref, what="unset full Z")
 
 
+@pytest.mark.parametrize("side,layers,image", [(1, 1, (1, 1, 1)), (1, 2, (1, 2, 3))])
+def test_smallest_tables_and_images(lib, side, layers, image):
+    """One crystal (no other crystal on any path), or one behind another; one voxel or a
+    ragged 1 x 2 x 3 image: empty candidate lists and tiles clamped to the image."""
+    g = params.tiny(seed=10, n_det_side=side, n_layers=layers, image=image, holes_side=1, n_plates=1)
+    gpu = pe_compute(g, tile=(4, 4, 4))
+    assert gpu.shape == (g.n_det, g.n_vox) and gpu.min() > 0
+    assert_parity(gpu, oracle.pe(g, precision="f64"), what=f"{g.n_det} det {image}")
+
+
 def test_zero_mu_detector_rows_are_zero(lib):
     g = params.tiny(seed=7, n_det_side=2, n_layers=1, image=(3, 3, 3), holes_side=2)
     g.crystals[1, 6] = 0.005
