@@ -72,6 +72,22 @@ def test_collimator_scatter_parity(lib, variant):
         assert_parity(gpu, ref, what=f"collimator v{variant} seed {seed}")
 
 
+@pytest.mark.parametrize("side,layers,image", [(1, 1, (1, 1, 1)), (1, 2, (1, 2, 3))])
+def test_smallest_tables_and_images(lib, side, layers, image):
+    """One crystal (no scatter site but itself: crystal scatter all zero) or two; one voxel or a
+    ragged 1 x 2 x 3 image; one hole."""
+    g = geo_small(10, side, layers, image=image, holes=1)
+    pe = oracle.pe(g, precision="f64")
+    bits, cbits = oracle.bitmap_crystal(g), oracle.bitmap_collimator(g)
+    assert np.array_equal(sc.bitmap_crystal(g), bits) and np.array_equal(sc.bitmap_collimator(g), cbits)
+    gpu = sc.scatter_crystal(g, pe, bits)
+    assert_parity(gpu, oracle.scatter_crystal(g, pe, bits, precision="f64"), what=f"crystal {g.n_det} det {image}")
+    gpu = sc.scatter_collimator(g, cbits)
+    ref = oracle.scatter_collimator(g, cbits, precision="f64")
+    assert ref.min() > 0
+    assert_parity(gpu, ref, what=f"collimator {g.n_det} det {image}")
+
+
 def test_rotation_and_window_variants(lib):
     g = geo_small(4, side=3, layers=2, image=(3, 4, 3))
     g.img[7] = 0.3
