@@ -70,6 +70,16 @@ inline std::string build_tables(const float* col, const float* det, float fov2c,
     T.n_det = count_of(det[0], 0.000001f);
     if (T.n_layers < 0 || T.n_layers > kMaxLayers) return "collimator layer count out of range (0..9)";
     if (T.n_det <= 0) return "detector count must be positive";
+    // the tracer keeps candidate crystal / hole indices in 16 bits; checked
+    // before any crystal or hole record is read
+    if (T.n_det > 65535) return "more than 65535 detector crystals";
+    long long total_holes = 0;
+    for (int m = 0; m < T.n_layers; ++m) {
+        const int nh = (int)std::floor(col[(m + 1) * 10]);
+        if (nh < 0) return "negative hole count";
+        total_holes += nh;
+    }
+    if (total_holes > 65535) return "more than 65535 collimator holes";
     std::vector<float> ys;  // distinct y-ranges, in pairs
     auto slab_id = [&](float y1, float y2) -> int {
         for (size_t i = 0; i + 1 < ys.size(); i += 2)
@@ -93,8 +103,7 @@ inline std::string build_tables(const float* col, const float* det, float fov2c,
         p.mu_pe = L[6];
         p.mu_c = L[7];
         p.slab = (float)slab_id(p.y1, p.y2);
-        int nh = (int)std::floor(L[0]);
-        if (nh < 0) return "negative hole count";
+        const int nh = (int)std::floor(L[0]);
         p.n_holes = (float)nh;
         p.hole_begin = h;
         p.hole_end = h + nh;
@@ -121,9 +130,6 @@ inline std::string build_tables(const float* col, const float* det, float fov2c,
         T.plates.push_back(p);
     }
     T.n_holes = h;
-    // the tracer keeps candidate crystal / hole indices in 16 bits
-    if (T.n_det > 65535) return "more than 65535 detector crystals";
-    if (T.n_holes > 65535) return "more than 65535 collimator holes";
     for (int d = 0; d < T.n_det; ++d) {
         const float* D = det + 12 * d;
         Crystal c{};
