@@ -2,6 +2,7 @@
 // file writes.  The drivers keep the reference's command lines, file names and
 // the raw float32 det-major .sysmat layout; errors exit non-zero with a message.
 #pragma once
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
 #include <cstdio>
@@ -49,6 +50,12 @@ inline Params read_all() {
     p.phy = read_params("Params_Physics.dat", 100);
     const int n = (int)std::floor(p.det[0] + 0.0001f);
     if (n <= 0 || p.det.size() < (size_t)(1 + 12 * n)) fail("Params_Detector.dat holds fewer crystals than its header");
+    const int layers = (int)std::floor(p.col[0] + 0.000001f);
+    if (layers < 0 || layers > 9) fail("Params_Collimator.dat: layer count outside 0..9");
+    long long holes = 0;
+    for (int m = 0; m < layers; ++m) holes += std::max(0LL, (long long)std::floor(p.col[(m + 1) * 10]));
+    if (p.col.size() < (size_t)(100 + 9 * holes))
+        fail("Params_Collimator.dat holds fewer holes than its layer headers");
     return p;
 }
 

@@ -47,6 +47,25 @@ def test_missing_params_file_is_an_error(name, tmp_path):
     assert r.returncode != 0 and "Params_" in r.stderr
 
 
+@pytest.mark.parametrize("name", DRIVERS)
+def test_inconsistent_collimator_headers_are_errors(name, tmp_path):
+    """Layer headers claiming more holes than Params_Collimator.dat holds (beyond the
+    80000-float padding), or more than 9 layers, stop the driver before any read past the data."""
+    g = params.tiny(seed=1)
+    g.write(str(tmp_path))
+    col = np.fromfile(tmp_path / "Params_Collimator.dat", np.float32)
+    bad = col.copy()
+    bad[10] = 10000  # 100 + 9 * 10000 floats needed
+    bad.tofile(tmp_path / "Params_Collimator.dat")
+    r = run(name, [], tmp_path)
+    assert r.returncode != 0 and "fewer holes" in r.stderr
+    bad = col.copy()
+    bad[0] = 12
+    bad.tofile(tmp_path / "Params_Collimator.dat")
+    r = run(name, [], tmp_path)
+    assert r.returncode != 0 and "layer count" in r.stderr
+
+
 def small_geometry():
     g = params.tiny(seed=7, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3)
     g.img[6] = 2  # two rotations
