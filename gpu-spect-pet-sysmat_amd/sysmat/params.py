@@ -47,6 +47,24 @@ def count(x: float, eps: float = 0.001) -> int:
     return int(np.floor(np.float32(x) + np.float32(eps)))
 
 
+def check_tables(col: np.ndarray, det: np.ndarray) -> None:
+    """Raise ValueError unless the collimator and detector arrays hold every record their
+    headers claim (at most MAX_LAYERS layers, 9 floats per hole after the 100-float layer
+    block, 12 per crystal): the library reads the records by the headers alone."""
+    if len(col) < HOLE_BASE:
+        raise ValueError(f"collimator array holds {len(col)} floats, the layer block needs {HOLE_BASE}")
+    n_layers = count(col[0], 1e-6)
+    if not 0 <= n_layers <= MAX_LAYERS:
+        raise ValueError(f"collimator layer count {n_layers} outside 0..{MAX_LAYERS}")
+    holes = sum(max(0, int(np.floor(col[(m + 1) * 10]))) for m in range(n_layers))
+    if len(col) < HOLE_BASE + HOLE_STRIDE * holes:
+        raise ValueError(f"collimator array holds {len(col)} floats, {holes} holes need "
+                         f"{HOLE_BASE + HOLE_STRIDE * holes}")
+    n = count(det[0], 1e-6) if len(det) else 0
+    if n <= 0 or len(det) < 1 + DET_STRIDE * n:
+        raise ValueError(f"detector array holds {len(det)} floats, its header claims {n} crystals")
+
+
 @dataclasses.dataclass
 class Geometry:
     """The four reference parameter arrays, as float32 numpy arrays."""

@@ -15,7 +15,7 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 from ._lib import SmPeOptions, check, load, ptr
-from .params import Geometry
+from .params import Geometry, check_tables
 
 
 def _f32(a) -> np.ndarray:
@@ -31,6 +31,7 @@ def _padded_img(img) -> np.ndarray:
 def PESysMatGen(parameter_Collimator, parameter_Detector, parameter_Image, dst: np.ndarray, cuda_id: int = 0) -> int:
     """Drop-in for the reference PESysMatGen; ``dst`` (float32, >= N*X*Y*Img[2]) is written in place."""
     col, det, img = _f32(parameter_Collimator), _f32(parameter_Detector), _padded_img(parameter_Image)
+    check_tables(col, det)
     if dst.dtype != np.float32 or not dst.flags.c_contiguous:
         raise ValueError("dst must be a C-contiguous float32 array")
     N = int(np.floor(det[0] + 1e-6))

@@ -16,7 +16,7 @@ from typing import Optional, Sequence, Tuple
 import numpy as np
 
 from ._lib import check, load, ptr
-from .params import Geometry
+from .params import Geometry, check_tables
 
 
 def _f32(a) -> np.ndarray:
@@ -87,14 +87,15 @@ def scatter_cells(geo: Geometry, cell_size: float = 0.5, keep_dist: float = 1.0,
 def scatter(parameter_Collimator, parameter_Detector, parameter_Image, parameter_Physics, PE_SysMat,
             FnameGeoCrystal: str, FnameGeoCollimator: str, dst: np.ndarray, cuda_id: int = 0) -> int:
     """Drop-in for the combined reference scatter(); ``dst`` (N*V float32) is written in place."""
-    det = _f32(parameter_Detector)
+    col, det = _f32(parameter_Collimator), _f32(parameter_Detector)
+    check_tables(col, det)
     img = _img(parameter_Image)
     N = int(np.floor(det[0] + 1e-4))
     V = int(np.floor(img[0] + 1e-3)) * int(np.floor(img[1] + 1e-3)) * int(np.floor(img[2] + 1e-3))
     pe = _f32(PE_SysMat)
     if pe.size < N * V or dst.size < N * V or dst.dtype != np.float32:
         raise ValueError("PE_SysMat and dst must hold N*V float32 values")
-    return check(load().sm_scatter(ptr(_f32(parameter_Collimator)), ptr(det), ptr(img), ptr(_f32(parameter_Physics)),
+    return check(load().sm_scatter(ptr(col), ptr(det), ptr(img), ptr(_f32(parameter_Physics)),
                                    ptr(pe), FnameGeoCrystal.encode(), FnameGeoCollimator.encode(), ptr(dst), cuda_id))
 
 

@@ -76,3 +76,21 @@ def test_config_shapes():
     vs = params.config4_variants(seed=0)
     assert len(vs) == 6 and all(v.n_det == 6912 for v in vs)
     assert 1 <= min(v.n_layers for v in vs) and max(v.n_layers for v in vs) <= 3
+
+
+def test_dropin_arrays_shorter_than_their_headers_are_rejected():
+    """The ctypes drop-ins check that the parameter arrays hold every record their headers
+    claim before the library reads them (no GPU call on these paths)."""
+    from sysmat import pe, scatter
+    g = params.tiny(seed=1)
+    dst = np.zeros(1, np.float32)
+    with pytest.raises(ValueError, match="holes need"):
+        pe.PESysMatGen(g.col[:100], g.det, g.img, dst)
+    with pytest.raises(ValueError, match="claims"):
+        pe.PESysMatGen(g.col, g.det[:13], g.img, dst)
+    bad = g.col.copy()
+    bad[0] = 10
+    with pytest.raises(ValueError, match="layer count"):
+        scatter.scatter(bad, g.det, g.img, g.phy, dst, "a", "b", dst)
+    params.check_tables(g.col, g.det)  # the generator's arrays are consistent
+    params.check_tables(params.config2().col, params.config2().det)
