@@ -71,7 +71,7 @@ def load_traffic(name="pe_tile_kernel_c2.md"):
     return float(m.group(1)), int(r.group(1)) if r else None
 
 
-def cpu_baseline(geo, nv, sub, row0, n_rows, plane, n_planes=4):
+def cpu_baseline(geo, nv, sub, row0, n_rows, plane, n_planes=10):
     """FP32 restatement of the reference kernel (oracle/, OpenMP over detector rows)
     timed on a bounded sample: n_rows consecutive detectors x n_planes voxel planes."""
     import oracle
@@ -501,8 +501,9 @@ def main():
         rng = np.random.default_rng(0)
         n_rows = args.cpu_rows if args.cpu_rows > 0 else min(64, max(8, os.cpu_count() or 8))
         row0 = int(rng.integers(0, N - n_rows))
-        plane = int(rng.integers(0, geo.shape[2] - 4))
-        cpu = cpu_baseline(geo, nv, sub, row0, n_rows, plane)
+        n_planes = 10  # about 10-15 s of host work at the measured rate
+        plane = int(rng.integers(0, geo.shape[2] - n_planes))
+        cpu = cpu_baseline(geo, nv, sub, row0, n_rows, plane, n_planes)
 
     rec = {
         "metric": METRIC,
