@@ -328,7 +328,16 @@ int sm_pe_compute(const float* col, const float* det, const float* img, const Sm
 int sm_PESysMatGen(const float* col, const float* det, const float* img, float* dst, int cuda_id) {
     clear_error();
     if (!col || !det || !img || !dst) return set_error(SM_E_ARG, "null argument");
-    // in-band slab description (PEGen_CircularHole.cpp:165,177,189-190)
+    // in-band slab description (PEGen_CircularHole.cpp:165,177,189-190).  The
+    // reference sizes the caller's buffer with floor(v + 0.001) (driver) and
+    // floor(v) (PESysMatGen.cu:396-405); the tables here use floor(v + 0.001).
+    // A count where the two differ (e.g. 63.9995) would make rows one voxel
+    // longer than the caller's buffer, so it is rejected.
+    for (int k : {0, 1, 2}) {
+        if (std::floor(img[k]) != std::floor(img[k] + 0.001f))
+            return set_error(SM_E_ARG, "image voxel count Img[" + std::to_string(k) + "] = " +
+                                           std::to_string(img[k]) + " is not integral");
+    }
     const int X = (int)std::floor(img[0]), Y = (int)std::floor(img[1]);
     const int slabZ = (int)std::floor(img[2]);
     const int fullZ = (int)std::floor(img[30]);
@@ -359,6 +368,10 @@ int sm_PESysMatGen(const float* col, const float* det, const float* img, float* 
             sm_pe_plan_destroy(plan);
             return rc;
         }
+    }
+    if (P->base.X != X || P->base.Y != Y || P->base.Zslab != slabZ) {
+        sm_pe_plan_destroy(plan);
+        return set_error(SM_E_ARG, "slab shape of the tables differs from the in-band slab fields");
     }
     const size_t n = (size_t)P->base.det_count * X * Y * slabZ;
     DeviceBuf<float> out;

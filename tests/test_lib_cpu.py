@@ -94,3 +94,21 @@ def test_dropin_arrays_shorter_than_their_headers_are_rejected():
         scatter.scatter(bad, g.det, g.img, g.phy, dst, "a", "b", dst)
     params.check_tables(g.col, g.det)  # the generator's arrays are consistent
     params.check_tables(params.config2().col, params.config2().det)
+
+
+def test_pesysmatgen_rejects_non_integral_voxel_counts():
+    """A count such as 63.9995 floors to 63 in the caller's buffer arithmetic (the reference
+    driver and PESysMatGen.cu:396-405) but to 64 with the +0.001 rounding of the tables: the
+    drop-in refuses it before any device call instead of writing rows longer than dst."""
+    L = _lib.load()
+    g = params.tiny(seed=1)
+    col, det = np.ascontiguousarray(g.col, np.float32), np.ascontiguousarray(g.det, np.float32)
+    for k in (0, 1, 2):
+        img = np.zeros(100, np.float32)
+        img[:len(g.img)] = g.img
+        img[30] = img[2]
+        img[k] = np.float32(img[k]) - np.float32(0.0005)
+        dst = np.zeros(1, np.float32)
+        rc = L.sm_PESysMatGen(col.ctypes.data, det.ctypes.data, img.ctypes.data, dst.ctypes.data, 0)
+        assert rc == _lib.SM_E_ARG
+        assert b"not integral" in L.sm_last_error()
