@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -39,8 +40,40 @@ struct PePlanImpl {
         hipStream_t stream;
     };
     std::vector<Launch> pending;  // launches since the last sync (re-run with smaller tiles on overflow)
+    DeviceArena* arena = nullptr; // set: every device buffer is a piece of this region (no allocation)
     PeLists lists() const { return PeLists{cry_ptr.ptr, cry_list.ptr, hol_ptr.ptr, hol_list.ptr}; }
 };
+
+// Device buffer of the plan: hipMalloc'd, or carved from the plan's arena.
+template <class T>
+static int plan_alloc(PePlanImpl& P, DeviceBuf<T>& b, size_t n) {
+    if (!P.arena) return b.alloc(n);
+    T* p = P.arena->take<T>(n);
+    if (!p)
+        return set_error(SM_E_ARG, "geometry tables and candidate lists exceed the " +
+                                       std::to_string(P.arena->cap >> 20) + " MiB allocation-free scratch");
+    b.attach(p, n);
+    return SM_OK;
+}
+
+template <class T>
+static int plan_upload(PePlanImpl& P, DeviceBuf<T>& b, const std::vector<T>& v) {
+    int rc = plan_alloc(P, b, v.size());
+    if (rc < 0) return rc;
+    if (!v.empty()) HIP_TRY(hipMemcpy(b.ptr, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return SM_OK;
+}
+
+__global__ void zero_ints_kernel(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+// Clears the status word with a kernel (no memset node if the caller captures the stream).
+static hipError_t clear_status(int* status, hipStream_t stream) {
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(1), dim3(64), 0, stream, status, 1);
+    return hipGetLastError();
+}
 
 static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, PeParams& p) {
     const int X = (int)std::floor(img[0] + 0.001f), Y = (int)std::floor(img[1] + 0.001f);
@@ -111,11 +144,11 @@ static int upload(PePlanImpl& P, const float* col, const float* det, const float
     std::string err = build_tables(col, det, img[11], P.T);
     if (!err.empty()) return set_error(SM_E_ARG, err);
     int rc;
-    if ((rc = P.cry.upload(P.T.crystals)) < 0) return rc;
-    if ((rc = P.hol.upload(P.T.holes)) < 0) return rc;
-    if ((rc = P.pla.upload(P.T.plates)) < 0) return rc;
-    if ((rc = P.slb.upload(P.T.slabs)) < 0) return rc;
-    if ((rc = P.status.alloc(1)) < 0) return rc;
+    if ((rc = plan_upload(P, P.cry, P.T.crystals)) < 0) return rc;
+    if ((rc = plan_upload(P, P.hol, P.T.holes)) < 0) return rc;
+    if ((rc = plan_upload(P, P.pla, P.T.plates)) < 0) return rc;
+    if ((rc = plan_upload(P, P.slb, P.T.slabs)) < 0) return rc;
+    if ((rc = plan_alloc(P, P.status, 1)) < 0) return rc;
     return SM_OK;
 }
 
@@ -126,9 +159,9 @@ static int build_lists(PePlanImpl& P) {
     const int n = p.n_det;
     DeviceBuf<int> counts;
     int rc;
-    if ((rc = counts.alloc(2 * (size_t)n)) < 0) return rc;
-    if ((rc = P.cry_ptr.alloc((size_t)n + 1)) < 0) return rc;
-    if ((rc = P.hol_ptr.alloc((size_t)n + 1)) < 0) return rc;
+    if ((rc = plan_alloc(P, counts, 2 * (size_t)n)) < 0) return rc;
+    if ((rc = plan_alloc(P, P.cry_ptr, (size_t)n + 1)) < 0) return rc;
+    if ((rc = plan_alloc(P, P.hol_ptr, (size_t)n + 1)) < 0) return rc;
     HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, P.slb.ptr, 0, counts.ptr, P.lists(), nullptr));
     std::vector<int> c(2 * (size_t)n);
     HIP_TRY(hipMemcpy(c.data(), counts.ptr, c.size() * sizeof(int), hipMemcpyDeviceToHost));
@@ -139,8 +172,8 @@ static int build_lists(PePlanImpl& P) {
     }
     HIP_TRY(hipMemcpy(P.cry_ptr.ptr, cp.data(), cp.size() * sizeof(long long), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(P.hol_ptr.ptr, hp.data(), hp.size() * sizeof(long long), hipMemcpyHostToDevice));
-    if ((rc = P.cry_list.alloc((size_t)cp[n])) < 0) return rc;
-    if ((rc = P.hol_list.alloc((size_t)hp[n])) < 0) return rc;
+    if ((rc = plan_alloc(P, P.cry_list, (size_t)cp[n])) < 0) return rc;
+    if ((rc = plan_alloc(P, P.hol_list, (size_t)hp[n])) < 0) return rc;
     HIP_TRY(launch_detector_cull(p, P.cry.ptr, P.hol.ptr, P.slb.ptr, 1, counts.ptr, P.lists(), nullptr));
     HIP_TRY(hipDeviceSynchronize());
     return SM_OK;
@@ -154,7 +187,7 @@ static int run_sync(PePlanImpl& P, PeParams p, float* dst, hipStream_t stream) {
     int g = kGroupTiles;
     for (;;) {
         set_tiles(p, t[0], t[1], t[2], g);
-        HIP_TRY(hipMemsetAsync(P.status.ptr, 0, sizeof(int), stream));
+        HIP_TRY(clear_status(P.status.ptr, stream));
         HIP_TRY(launch_pe(p, P.cry.ptr, P.hol.ptr, P.pla.ptr, P.slb.ptr, P.lists(), dst, P.status.ptr, stream));
         int st = 0;
         HIP_TRY(hipMemcpyAsync(&st, P.status.ptr, sizeof(int), hipMemcpyDeviceToHost, stream));
@@ -174,6 +207,56 @@ static void auto_tiles(const SmPeOptions& o, int& tx, int& ty, int& tz) {
     tx = o.tile_x > 0 ? o.tile_x : 4;
     ty = o.tile_y > 0 ? o.tile_y : 4;
     tz = o.tile_z > 0 ? o.tile_z : 4;
+}
+
+// Plan on the current device: tables, candidate lists and the status word in
+// hipMalloc'd buffers, or (arena != nullptr) carved from that region.
+static int plan_create(const float* col, const float* det, const float* img, const SmPeOptions& o,
+                       DeviceArena* arena, PePlanImpl** out) {
+    auto* P = new PePlanImpl();
+    P->device = o.device;
+    P->arena = arena;
+    int rc = upload(*P, col, det, img);
+    if (rc >= 0) rc = fill_params(img, o, P->T, P->base);
+    if (rc >= 0) rc = build_lists(*P);
+    if (rc >= 0 && (clear_status(P->status.ptr, nullptr) != hipSuccess || hipStreamSynchronize(nullptr) != hipSuccess))
+        rc = set_error(SM_E_DEVICE, "status reset failed");
+    if (rc < 0) {
+        delete P;
+        return rc;
+    }
+    int tx, ty, tz;
+    auto_tiles(o, tx, ty, tz);
+    set_tiles(P->base, tx, ty, tz);
+    P->base.det_begin = o.det_begin;
+    P->base.det_count = o.det_count > 0 ? o.det_count : P->T.n_det - o.det_begin;
+    *out = P;
+    return SM_OK;
+}
+
+// ---- allocation-free variant of sm_pe_compute_device -----------------------
+// Every device buffer of its plan (geometry tables, per-detector candidate
+// lists, status word) is a piece of this file-scope array; each device holds
+// its own copy, whose address is looked up once per device.
+constexpr size_t kNoAllocScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kNoAllocScratchBytes];
+constexpr int kScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static unsigned char* g_scratch_ptr[kScratchDevices];
+
+// Device address of `scratch` on `device` (which must be current).
+static int scratch_region(int device, unsigned char** out) {
+    if (device < 0 || device >= kScratchDevices) return set_error(SM_E_DEVICE, "device id out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return set_error(SM_E_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+        g_scratch_ptr[device] = static_cast<unsigned char*>(p);
+    }
+    *out = g_scratch_ptr[device];
+    return SM_OK;
 }
 
 }  // namespace sysmat
@@ -204,24 +287,8 @@ int sm_pe_plan_create(const float* col, const float* det, const float* img, cons
     if (!col || !det || !img || !o || !plan) return set_error(SM_E_ARG, "null argument");
     int rc = use_device(o->device);
     if (rc < 0) return rc;
-    auto* P = new PePlanImpl();
-    P->device = o->device;
-    rc = upload(*P, col, det, img);
-    if (rc >= 0) rc = fill_params(img, *o, P->T, P->base);
-    if (rc >= 0) rc = build_lists(*P);
-    if (rc < 0) {
-        delete P;
-        return rc;
-    }
-    int tx, ty, tz;
-    auto_tiles(*o, tx, ty, tz);
-    set_tiles(P->base, tx, ty, tz);
-    if (hipMemset(P->status.ptr, 0, sizeof(int)) != hipSuccess) {
-        delete P;
-        return set_error(SM_E_DEVICE, "status reset failed");
-    }
-    P->base.det_begin = o->det_begin;
-    P->base.det_count = o->det_count > 0 ? o->det_count : P->T.n_det - o->det_begin;
+    PePlanImpl* P = nullptr;
+    if ((rc = plan_create(col, det, img, *o, nullptr, &P)) < 0) return rc;
     *plan = reinterpret_cast<SmPePlan*>(P);
     return SM_OK;
 }
@@ -302,6 +369,32 @@ int sm_pe_compute_device(const float* col, const float* det, const float* img, c
     }
     rc = run_sync(*P, p, dst_dev, (hipStream_t)stream);
     sm_pe_plan_destroy(plan);
+    return rc;
+}
+
+long long sm_pe_noalloc_scratch_bytes(void) { return (long long)kNoAllocScratchBytes; }
+
+int sm_pe_compute_device_noalloc(const float* col, const float* det, const float* img, const SmPeOptions* o,
+                                 float* dst_dev, long long row_stride, void* stream) {
+    clear_error();
+    if (!col || !det || !img || !o || !dst_dev) return set_error(SM_E_ARG, "null argument");
+    int rc = use_device(o->device);
+    if (rc < 0) return rc;
+    unsigned char* base = nullptr;
+    if ((rc = scratch_region(o->device, &base)) < 0) return rc;
+    DeviceArena arena;
+    arena.base = base;
+    arena.cap = kNoAllocScratchBytes;
+    PePlanImpl* P = nullptr;
+    if ((rc = plan_create(col, det, img, *o, &arena, &P)) < 0) return rc;
+    PeParams p = P->base;
+    p.row_stride = row_stride > 0 ? row_stride : (long long)p.X * p.Y * p.Zslab;
+    if (p.det_begin < 0 || p.det_count <= 0 || p.det_begin + p.det_count > P->T.n_det) {
+        delete P;
+        return set_error(SM_E_ARG, "detector range outside the detector table");
+    }
+    rc = run_sync(*P, p, dst_dev, (hipStream_t)stream);
+    delete P;  // views of the scratch array only: nothing is freed
     return rc;
 }
 

@@ -25,14 +25,22 @@ template <class T>
 struct DeviceBuf {
     T* ptr = nullptr;
     size_t n = 0;
+    bool owned = true;  // false: a view of memory someone else owns (never freed here)
     DeviceBuf() = default;
     DeviceBuf(const DeviceBuf&) = delete;
     DeviceBuf& operator=(const DeviceBuf&) = delete;
     ~DeviceBuf() { release(); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && owned) (void)hipFree(ptr);
         ptr = nullptr;
         n = 0;
+        owned = true;
+    }
+    void attach(T* p, size_t count) {
+        release();
+        ptr = p;
+        n = count;
+        owned = false;
     }
     int alloc(size_t count) {
         release();
@@ -43,6 +51,7 @@ struct DeviceBuf {
             return set_error(SM_E_NOMEM, std::string("hipMalloc: ") + hipGetErrorString(e));
         }
         n = count;
+        owned = true;
         return 0;
     }
     int upload(const std::vector<T>& v) {
@@ -53,6 +62,21 @@ struct DeviceBuf {
             if (e != hipSuccess) return set_error(SM_E_DEVICE, std::string("hipMemcpy: ") + hipGetErrorString(e));
         }
         return 0;
+    }
+};
+
+// Bump allocator over a device region that someone else owns (no allocation):
+// consecutive 256-byte-aligned pieces, nullptr once the region is full.
+struct DeviceArena {
+    unsigned char* base = nullptr;
+    size_t cap = 0, used = 0;
+    template <class T>
+    T* take(size_t count) {
+        if (count == 0) count = 1;
+        const size_t off = (used + 255) & ~size_t(255);
+        if (off > cap || count > (cap - off) / sizeof(T)) return nullptr;
+        used = off + count * sizeof(T);
+        return reinterpret_cast<T*>(base + off);
     }
 };
 

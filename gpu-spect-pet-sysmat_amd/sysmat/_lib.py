@@ -49,6 +49,9 @@ SIGNATURES = {
     "sm_pe_compute": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), _VP]),
     "sm_pe_compute_device": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), _VP, ctypes.c_longlong,
                                             _VP]),
+    "sm_pe_noalloc_scratch_bytes": (ctypes.c_longlong, []),
+    "sm_pe_compute_device_noalloc": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), _VP,
+                                                    ctypes.c_longlong, _VP]),
     "sm_pe_autotune": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), ctypes.c_int, ctypes.c_char_p,
                                       ctypes.POINTER(SmPeOptions), ctypes.POINTER(ctypes.c_double)]),
     "sm_pe_plan_create": (ctypes.c_int, [_VP, _VP, _VP, ctypes.POINTER(SmPeOptions), ctypes.POINTER(_VP)]),

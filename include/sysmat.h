@@ -72,6 +72,20 @@ int sm_pe_compute(const float* col, const float* det, const float* img, const Sm
 int sm_pe_compute_device(const float* col, const float* det, const float* img, const SmPeOptions* o,
                          float* dst_dev, long long row_stride, void* stream);
 
+/* Allocation-free form of sm_pe_compute_device.  No counterpart in the
+ * reference (whose PESysMatGen allocates per call, PESysMatGen.cu:792-804); it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  The caller
+ * passes inputs and dst_dev; the geometry tables, per-detector candidate lists
+ * and status word live in a file-scope device array of
+ * sm_pe_noalloc_scratch_bytes() (64 MiB) per GPU, held until the process ends.
+ * A geometry whose tables and lists exceed it returns SM_E_ARG.  Calls share
+ * that array: calls that overlap on one GPU (several host threads) race.
+ * Results equal sm_pe_compute_device's bit for bit.  Blocks until done. */
+long long sm_pe_noalloc_scratch_bytes(void);
+int sm_pe_compute_device_noalloc(const float* col, const float* det, const float* img, const SmPeOptions* o,
+                                 float* dst_dev, long long row_stride, void* stream);
+
 /* Tile-shape autotuner (no counterpart in the reference): times candidate voxel
  * tiles on ~sample_rows detector rows spread over the detector (sampling and
  * image as in `base`) and returns `base` with the fastest tile in `best`.
