@@ -454,7 +454,8 @@ def main():
     avg_kernel_s = sum(pe_ms) / len(pe_ms) / 1e3
     log(f"C2 PE+CSR: {value:.3e} elements/s, PE {np.mean(pe_ms):.1f} ms, CSR {np.mean(csr_ms):.1f} ms per step")
 
-    fpe = load_flops("c2_pe_nv3_sub222")
+    fpe = load_flops("c2_pe_nv3_sub222_touched")
+    fpe_ref = load_flops("c2_pe_nv3_sub222")
     roof = None
     if fpe:
         achieved = fpe * B * V / avg_kernel_s / 1e12
@@ -467,9 +468,15 @@ def main():
                 "flops_per_element": round(fpe, 1), "kernel": "pe_tile_kernel",
                 "avg_launch_ms": round(avg_kernel_s * 1e3, 3), "rows_per_launch": B,
                 "algorithmic_output_bytes_per_launch": B * V * 4,
-                "flops_basis": "FP32 operations of the reference algorithm per element (op-counting restatement "
-                               "of photodetectorCudaMe, every crystal its cull keeps); the tracer skips crystals "
-                               "and holes whose chord is provably zero for every sub-ray, so frac can exceed 1",
+                "flops_basis": "FP32 operations of the reference algorithm (op-counting restatement of "
+                               "photodetectorCudaMe) on the culled crystals and holes that touch the element "
+                               "(non-zero chord on some sub-ray); the others add exactly zero",
+                "flops_per_element_reference_basis": round(fpe_ref, 1) if fpe_ref else None,
+                "achieved_reference_basis": round(fpe_ref * B * V / avg_kernel_s / 1e12, 3) if fpe_ref else None,
+                "non_packed_peak": FP32_PEAK_TFLOPS / 2,
+                "packed_fp32": False,
+                "executed_fp32_tflops": load_profile_figure("executed_fp32_tflops"),
+                "executed_fp64_tflops": load_profile_figure("executed_fp64_tflops"),
                 "valu_issue_frac": load_profile_figure("valu_issue_frac")}
 
     if args.calib_copy:

@@ -65,7 +65,7 @@ def lib():
 
 class ScOpts(ctypes.Structure):
     _fields_ = [("det_begin", ctypes.c_int), ("det_end", ctypes.c_int), ("threads", ctypes.c_int),
-                ("variant", ctypes.c_int)]
+                ("variant", ctypes.c_int), ("vox", ctypes.c_void_p), ("n_vox", ctypes.c_longlong)]
 
 
 def _init_scatter(L) -> None:
@@ -76,8 +76,9 @@ def _init_scatter(L) -> None:
     L.or_collimator_bitmap_words.restype = ll
     L.or_bitmap_crystal.argtypes = [_f32p, _u32p, ll]
     L.or_bitmap_collimator.argtypes = [_f32p, _f32p, _u32p, ll]
-    L.or_scatter_crystal.argtypes = [_f32p, _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.POINTER(ScOpts), i]
-    L.or_scatter_collimator.argtypes = [_f32p, _f32p, _f32p, _f32p, _u32p, _f32p, ctypes.POINTER(ScOpts), i]
+    L.or_crystal_bits_for_pairs.argtypes = [_f32p, vp, vp, i, vp]
+    L.or_scatter_crystal.argtypes = [_f32p, _f32p, _f32p, _f32p, vp, _f32p, ctypes.POINTER(ScOpts), i]
+    L.or_scatter_collimator.argtypes = [_f32p, _f32p, _f32p, _f32p, vp, _f32p, ctypes.POINTER(ScOpts), i]
     L.or_cells.argtypes = [_f32p, ctypes.c_float, ctypes.c_double, ctypes.c_double, vp, ll]
     L.or_cells.restype = ll
     L.or_plate_path_f64.argtypes = [_f32p, ctypes.c_float, _f64p, _f64p]
@@ -119,32 +120,59 @@ def _sc_img(geo, rot):
     return img
 
 
-def scatter_crystal(geo, pe_sysmat: np.ndarray, bits: np.ndarray, det_range=None, precision="f32", rot=0,
-                    threads=0) -> np.ndarray:
-    """crystalScatterSysMatCuda rows (scatter.cu:1077-1437); pe_sysmat is the N x V PE matrix."""
+def _vox(vox):
+    if vox is None:
+        return None, None, 0
+    v = np.ascontiguousarray(vox, np.int32)
+    return v, v.ctypes.data, int(v.size)
+
+
+def scatter_crystal(geo, pe_sysmat: np.ndarray, bits: Optional[np.ndarray], det_range=None, precision="f32", rot=0,
+                    threads=0, vox=None) -> np.ndarray:
+    """crystalScatterSysMatCuda rows (scatter.cu:1077-1437); pe_sysmat is the N x V PE matrix,
+    or N x len(vox) columns when the voxel sample ``vox`` is given (rows then hold those voxels).
+    bits None: the GeometryRelationShip criterion is evaluated inline (same lists)."""
     b, e = det_range if det_range is not None else (0, geo.n_det)
-    V = geo.n_vox
+    keep, vp, nv = _vox(vox)
+    V = nv if vox is not None else geo.n_vox
     dst = np.zeros((e - b) * V, np.float32)
-    o = ScOpts(b, e, threads, 0)
+    o = ScOpts(b, e, threads, 0, vp, nv)
+    bw = None if bits is None else np.ascontiguousarray(bits, np.uint32)
     lib().or_scatter_crystal(np.ascontiguousarray(geo.det, np.float32), _sc_img(geo, rot),
                              np.ascontiguousarray(geo.phy, np.float32),
-                             np.ascontiguousarray(pe_sysmat, np.float32).ravel(), np.ascontiguousarray(bits), dst,
-                             ctypes.byref(o), 1 if precision == "f64" else 0)
+                             np.ascontiguousarray(pe_sysmat, np.float32).ravel(),
+                             None if bw is None else bw.ctypes.data, dst, ctypes.byref(o),
+                             1 if precision == "f64" else 0)
     return dst.reshape(e - b, V)
 
 
-def scatter_collimator(geo, bits: np.ndarray, det_range=None, precision="f32", variant=0, rot=0,
-                       threads=0) -> np.ndarray:
+def scatter_collimator(geo, bits: Optional[np.ndarray], det_range=None, precision="f32", variant=0, rot=0,
+                       threads=0, vox=None) -> np.ndarray:
     """collimatorScatterSysMatCuda rows (variant 0: scatter.cu:1544-1936; variant 1: the
-    ScatterGen_Collimator kernel, ScatterGen_Collimator/scatter.cu:1056-1436)."""
+    ScatterGen_Collimator kernel, ScatterGen_Collimator/scatter.cu:1056-1436); voxel sample
+    and bits None as scatter_crystal."""
     b, e = det_range if det_range is not None else (0, geo.n_det)
-    V = geo.n_vox
+    keep, vp, nv = _vox(vox)
+    V = nv if vox is not None else geo.n_vox
     dst = np.zeros((e - b) * V, np.float32)
-    o = ScOpts(b, e, threads, variant)
+    o = ScOpts(b, e, threads, variant, vp, nv)
+    bw = None if bits is None else np.ascontiguousarray(bits, np.uint32)
     lib().or_scatter_collimator(np.ascontiguousarray(geo.col, np.float32), np.ascontiguousarray(geo.det, np.float32),
                                 _sc_img(geo, rot), np.ascontiguousarray(geo.phy, np.float32),
-                                np.ascontiguousarray(bits), dst, ctypes.byref(o), 1 if precision == "f64" else 0)
+                                None if bw is None else bw.ctypes.data, dst, ctypes.byref(o),
+                                1 if precision == "f64" else 0)
     return dst.reshape(e - b, V)
+
+
+def crystal_bits_for_pairs(geo, pairs) -> np.ndarray:
+    """(len(pairs), N) uint8: bit (pair(i, j), k) of the crystal bitmap for each (i < j)."""
+    p = np.ascontiguousarray(pairs, np.int32).reshape(-1, 2)
+    assert np.all(p[:, 0] < p[:, 1])
+    pi, pj = np.ascontiguousarray(p[:, 0]), np.ascontiguousarray(p[:, 1])
+    out = np.zeros((len(p), geo.n_det), np.uint8)
+    lib().or_crystal_bits_for_pairs(np.ascontiguousarray(geo.det, np.float32), pi.ctypes.data, pj.ctypes.data,
+                                    len(p), out.ctypes.data)
+    return out
 
 
 def cells(geo, h: float, keep: float) -> np.ndarray:
@@ -220,18 +248,20 @@ def pe(geo, nv: int = 1, sub: Tuple[int, int, int] = (8, 16, 8), det_range=None,
 
 
 def pe_count(geo, nv: int = 1, sub=(8, 16, 8), det_range=None, threads: int = 0, z0: int = 0,
-             slab_z: Optional[int] = None):
-    """(values, per-row stats[rows, 3]) with stats = counted FLOPs, culled crystals, culled holes
-    (sums over the row's elements of the slab)."""
+             slab_z: Optional[int] = None, touched: bool = False):
+    """(values, per-row stats[rows, 5]) with stats = counted FLOPs, culled crystals, culled holes,
+    and (touched=True) culled crystals / holes with a non-zero chord on some sub-ray of their
+    element (sums over the row's elements of the slab).  touched=True counts only the work on
+    those attenuators (the others add exactly zero); False counts every culled attenuator."""
     img = slab_image(geo.img, z0, slab_z)
     bins = int(np.floor(img[0])) * int(np.floor(img[1])) * int(np.floor(img[2]))
-    o = _opts(nv, sub, det_range, geo.n_det, threads)
+    o = _opts(nv, sub, det_range, geo.n_det, threads, 64 if touched else 0)
     rows = o.det_end - o.det_begin
     dst = np.zeros(rows * bins, np.float32)
-    stats = np.zeros(rows * 3, np.float64)
+    stats = np.zeros(rows * 5, np.float64)
     lib().or_pe_count(np.ascontiguousarray(geo.col, np.float32), np.ascontiguousarray(geo.det, np.float32), img, dst,
                       ctypes.byref(o), stats)
-    return dst.reshape(rows, bins), stats.reshape(rows, 3)
+    return dst.reshape(rows, bins), stats.reshape(rows, 5)
 
 
 def chord_box(args, precision="f32", open_end=False) -> float:

@@ -30,39 +30,64 @@ struct ScOpts {
     int det_begin, det_end;  // detector rows
     int threads;             // <= 0: all
     int variant;             // collimator: 0 combined (ScatterGen_RayTracing_CircularHole), 1 ScatterGen_Collimator
+    const int* vox;          // optional voxel sample: rows hold these voxels only, and the crystal
+    long long n_vox;         // scatter's PE input holds these columns only (N x n_vox)
 };
 
 static inline int nfloor(float v, float eps) { return (int)std::floor(v + eps); }
 
-// ---- bitmap builders (FP32, as the reference kernels) ----------------------
+// ---- bitmap criteria and builders (FP32, as the reference kernels) ----------
+// Crystal k attenuates the pair (i, j), i < j: its bounding sphere, seen from i
+// or from j, overlaps the other crystal of the pair (scatter.cu:1440-1540).
+static bool crystal_criterion(const float* det, int i, int j, int k) {
+    const float* di = det + 12 * i;
+    const float* dj = det + 12 * j;
+    const float* dk = det + 12 * k;
+    float Ri = std::sqrt(di[4] * di[4] + di[6] * di[6] + di[5] * di[5]) / 2.0f;
+    float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
+    float Lij = distance<float>(di[1], di[2], di[3], dj[1], dj[2], dj[3]);
+    float cij = Ri / Lij, cji = Rj / Lij;
+    float uijx = (di[1] - dj[1]) / Lij, uijy = (di[2] - dj[2]) / Lij, uijz = (di[3] - dj[3]) / Lij;
+    float ujix = (dj[1] - di[1]) / Lij, ujiy = (dj[2] - di[2]) / Lij, ujiz = (dj[3] - di[3]) / Lij;
+    float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
+    float Lik = distance<float>(dk[1], dk[2], dk[3], di[1], di[2], di[3]);
+    float cki = Rk / Lik;
+    float Ljk = distance<float>(dk[1], dk[2], dk[3], dj[1], dj[2], dj[3]);
+    float ckj = Rk / Ljk;
+    float ukix = (dk[1] - di[1]) / Lik, ukiy = (dk[2] - di[2]) / Lik, ukiz = (dk[3] - di[3]) / Lik;
+    float ukjx = (dk[1] - dj[1]) / Ljk, ukjy = (dk[2] - dj[2]) / Ljk, ukjz = (dk[3] - dj[3]) / Ljk;
+    bool cross = distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji;
+    if (!cross) cross = distance<float>(ukjx, ukjy, ukjz, uijx, uijy, uijz) <= ckj + cij;
+    return cross;
+}
+
+// Crystal k attenuates the path hole h -> crystal j (scatter.cu:1939-2009).
+static bool collimator_criterion(const float* col, const float* det, int h, int j, int k) {
+    const float* hp = col + 100 + 9 * h;
+    float xh = hp[0], yh = (hp[1] + hp[2]) / 2.0f, zh = hp[3];
+    const float* dj = det + 12 * j;
+    float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
+    float Lij = distance<float>(xh, yh, zh, dj[1], dj[2], dj[3]);
+    float cji = Rj / Lij;
+    float ujix = (dj[1] - xh) / Lij, ujiy = (dj[2] - yh) / Lij, ujiz = (dj[3] - zh) / Lij;
+    const float* dk = det + 12 * k;
+    float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
+    float Lik = distance<float>(dk[1], dk[2], dk[3], xh, yh, zh);
+    float cki = Rk / Lik;
+    float ukix = (dk[1] - xh) / Lik, ukiy = (dk[2] - yh) / Lik, ukiz = (dk[3] - zh) / Lik;
+    return distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji;
+}
+
 static void build_crystal_bitmap(const float* det, uint32_t* words, long long n_words) {
     const int n = (int)det[0];
     std::memset(words, 0, sizeof(uint32_t) * (size_t)n_words);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int i = 0; i < n; ++i) {
-        const float* di = det + 12 * i;
-        float Ri = std::sqrt(di[4] * di[4] + di[6] * di[6] + di[5] * di[5]) / 2.0f;
         for (int j = i + 1; j < n; ++j) {
-            const float* dj = det + 12 * j;
-            float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
-            float Lij = distance<float>(di[1], di[2], di[3], dj[1], dj[2], dj[3]);
-            float cij = Ri / Lij, cji = Rj / Lij;
-            float uijx = (di[1] - dj[1]) / Lij, uijy = (di[2] - dj[2]) / Lij, uijz = (di[3] - dj[3]) / Lij;
-            float ujix = (dj[1] - di[1]) / Lij, ujiy = (dj[2] - di[2]) / Lij, ujiz = (dj[3] - di[3]) / Lij;
             long long pair = (long long)i * n - ((long long)i * (i + 1)) / 2 + j;
             for (int k = 0; k < n; ++k) {
                 if (k == i || k == j) continue;
-                const float* dk = det + 12 * k;
-                float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
-                float Lik = distance<float>(dk[1], dk[2], dk[3], di[1], di[2], di[3]);
-                float cki = Rk / Lik;
-                float Ljk = distance<float>(dk[1], dk[2], dk[3], dj[1], dj[2], dj[3]);
-                float ckj = Rk / Ljk;
-                float ukix = (dk[1] - di[1]) / Lik, ukiy = (dk[2] - di[2]) / Lik, ukiz = (dk[3] - di[3]) / Lik;
-                float ukjx = (dk[1] - dj[1]) / Ljk, ukjy = (dk[2] - dj[2]) / Ljk, ukjz = (dk[3] - dj[3]) / Ljk;
-                bool cross = distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji;
-                if (!cross) cross = distance<float>(ukjx, ukjy, ukjz, uijx, uijy, uijz) <= ckj + cij;
-                if (cross) {
+                if (crystal_criterion(det, i, j, k)) {
                     long long bit = pair * n + k;
 #pragma omp atomic
                     words[bit >> 5] |= (1u << (bit & 31));
@@ -78,21 +103,9 @@ static void build_collimator_bitmap(const float* col, const float* det, uint32_t
     std::memset(words, 0, sizeof(uint32_t) * (size_t)n_words);
 #pragma omp parallel for schedule(dynamic, 1)
     for (int h = 0; h < nh; ++h) {
-        const float* hp = col + 100 + 9 * h;
-        float xh = hp[0], yh = (hp[1] + hp[2]) / 2.0f, zh = hp[3];
         for (int j = 0; j < n; ++j) {
-            const float* dj = det + 12 * j;
-            float Rj = std::sqrt(dj[4] * dj[4] + dj[6] * dj[6] + dj[5] * dj[5]) / 2.0f;
-            float Lij = distance<float>(xh, yh, zh, dj[1], dj[2], dj[3]);
-            float cji = Rj / Lij;
-            float ujix = (dj[1] - xh) / Lij, ujiy = (dj[2] - yh) / Lij, ujiz = (dj[3] - zh) / Lij;
             for (int k = 0; k < n; ++k) {
-                const float* dk = det + 12 * k;
-                float Rk = std::sqrt(dk[4] * dk[4] + dk[6] * dk[6] + dk[5] * dk[5]) / 2.0f;
-                float Lik = distance<float>(dk[1], dk[2], dk[3], xh, yh, zh);
-                float cki = Rk / Lik;
-                float ukix = (dk[1] - xh) / Lik, ukiy = (dk[2] - yh) / Lik, ukiz = (dk[3] - zh) / Lik;
-                if (distance<float>(ukix, ukiy, ukiz, ujix, ujiy, ujiz) <= cki + cji) {
+                if (collimator_criterion(col, det, h, j, k)) {
                     long long bit = collimator_bit(h, j, k, n);
 #pragma omp atomic
                     words[bit >> 5] |= (1u << (bit & 31));
@@ -269,7 +282,10 @@ static void voxel_point(const float* img, int vox, bool eps_counts, R& x, R& y, 
     z = zi;
 }
 
-// crystalScatterSysMatCuda over rows [det_begin, det_end) x all voxels.
+// crystalScatterSysMatCuda over rows [det_begin, det_end) x all voxels (or the
+// voxel sample o.vox, with pe holding those columns only).  bits == nullptr:
+// the GeometryRelationShip criterion is evaluated per (pair, crystal) instead of
+// read from the bitmap (the same FP32 test, so the same attenuator lists).
 template <class R>
 static void crystal_run(const float* det, const float* img, const float* phy, const float* pe, const uint32_t* bits,
                         float* dst, const ScOpts& o) {
@@ -286,11 +302,16 @@ static void crystal_run(const float* det, const float* img, const float* phy, co
         std::vector<std::vector<int>> lists(n);
         for (int k = 0; k < n; ++k) {
             if (k == r) continue;
-            for (int a = 0; a < n; ++a)
-                if (a != r && a != k && test_bit(bits, crystal_pair_bit(k, r, a, n))) lists[k].push_back(a);
+            for (int a = 0; a < n; ++a) {
+                if (a == r || a == k) continue;
+                const bool on = bits ? test_bit(bits, crystal_pair_bit(k, r, a, n)) != 0
+                                     : crystal_criterion(det, std::min(k, r), std::max(k, r), a);
+                if (on) lists[k].push_back(a);
+            }
         }
-        const R ratio_c = D.mu_c / D.mu_pe;  // mu_c / mu_pe of the detecting crystal (scatter.cu:1218-1219)
-        for (long v = 0; v < V; ++v) {
+        const long nv = o.vox ? (long)o.n_vox : V;
+        for (long j = 0; j < nv; ++j) {
+            const long v = o.vox ? (long)o.vox[j] : j;
             R acc = R(0.0f);
             if (use && D.mu_t > R(0.01f)) {
                 R xI, yI, zI;
@@ -298,18 +319,19 @@ static void crystal_run(const float* det, const float* img, const float* phy, co
                 for (int k = 0; k < n; ++k) {
                     if (k == r) continue;
                     const float* dk = det + 12 * k;
-                    const R w = R(pe[(long)k * V + v]) * D.mu_c / D.mu_pe;
-                    (void)ratio_c;
+                    // mu_c / mu_pe of the detecting crystal (scatter.cu:1218-1219)
+                    const R w = R(pe[(long)k * nv + j]) * D.mu_c / D.mu_pe;
                     acc += site_term<R>(D, det, img, xI, yI, zI, R(dk[1]), R(dk[2]) + R(img[11]), R(dk[3]), w,
                                         lists[k]);
                 }
             }
-            dst[(long)rr * V + v] = (float)to_double(acc);
+            dst[(long)rr * nv + j] = (float)to_double(acc);
         }
     }
 }
 
-// collimatorScatterSysMatCuda (variant 0) / ScatterGen_Collimator (variant 1).
+// collimatorScatterSysMatCuda (variant 0) / ScatterGen_Collimator (variant 1);
+// voxel sample and bits == nullptr as crystal_run.
 template <class R>
 static void collimator_run(const float* col, const float* det, const float* img, const float* phy,
                            const uint32_t* bits, float* dst, const ScOpts& o) {
@@ -329,9 +351,15 @@ static void collimator_run(const float* col, const float* det, const float* img,
         Row<R> D = make_row<R>(det, img, phy, r);
         std::vector<std::vector<int>> lists(nh);
         for (int h = 0; h < nh; ++h)
-            for (int a = 0; a < n; ++a)
-                if (a != r && test_bit(bits, collimator_bit(h, r, a, n))) lists[h].push_back(a);
-        for (long v = 0; v < V; ++v) {
+            for (int a = 0; a < n; ++a) {
+                if (a == r) continue;
+                const bool on = bits ? test_bit(bits, collimator_bit(h, r, a, n)) != 0
+                                     : collimator_criterion(col, det, h, r, a);
+                if (on) lists[h].push_back(a);
+            }
+        const long nv = o.vox ? (long)o.n_vox : V;
+        for (long j = 0; j < nv; ++j) {
+            const long v = o.vox ? (long)o.vox[j] : j;
             R acc = R(0.0f);
             if (use && D.mu_t > R(0.01f)) {
                 R xI, yI, zI;
@@ -356,7 +384,7 @@ static void collimator_run(const float* col, const float* det, const float* img,
                     acc += site_term<R>(D, det, img, xI, yI, zI, xS, yS, zS, w, lists[h]);
                 }
             }
-            dst[(long)rr * V + v] = (float)to_double(acc);
+            dst[(long)rr * nv + j] = (float)to_double(acc);
         }
     }
 }
@@ -583,6 +611,18 @@ long long or_collimator_bitmap_words(int n_holes, int n) { return ((long long)n 
 
 int or_bitmap_crystal(const float* det, uint32_t* words, long long n_words) {
     oracle::build_crystal_bitmap(det, words, n_words);
+    return 0;
+}
+
+// The crystal-bitmap bits of listed pairs (i[p] < j[p]): out[p * n + k] = bit
+// (pair(i, j), k) of the GeometryRelationShip_Crystal2Crystal layout.
+int or_crystal_bits_for_pairs(const float* det, const int* pi, const int* pj, int n_pairs, unsigned char* out) {
+    const int n = (int)det[0];
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int p = 0; p < n_pairs; ++p)
+        for (int k = 0; k < n; ++k)
+            out[(long long)p * n + k] =
+                (k != pi[p] && k != pj[p] && oracle::crystal_criterion(det, pi[p], pj[p], k)) ? 1 : 0;
     return 0;
 }
 

@@ -5,7 +5,17 @@ oracle/pe_oracle.cpp instantiated with Counted<float>): every + - * / sqrt exp
 counts 1; comparisons, fabs, min/max, conversions and the sin/cos of fixed
 rotation angles count 0; the per-element culling prologue
 (PESysMatGen.cu:504-590) is excluded.  Counted exhaustively over seeded
-samples of (detector row, voxel plane) pairs.
+samples of (detector row, voxel plane) pairs, the same number of rows from each
+detector layer.
+
+Two bases per workload:
+  flops_per_element          every crystal and hole the reference's per-element
+                             cull keeps (PESysMatGen.cu:687-725, 661-685), the
+                             reference algorithm's work as written;
+  flops_per_element_touched  only the culled crystals and holes with a non-zero
+                             chord on at least one sub-ray of the element -- the
+                             others add exactly zero to the attenuation, so this is
+                             the work the result needs (the value is unchanged).
 
 Run: python tests/golden/make_flops.py
 """
@@ -23,22 +33,33 @@ import oracle  # noqa: E402
 from sysmat import params  # noqa: E402
 
 
-def sample(geo, nv, sub, n_rows, seed):
+def stratified(geo, per_layer, seed):
+    y = np.asarray(geo.det[1:1 + 12 * geo.n_det], np.float64).reshape(geo.n_det, 12)[:, 1]
     rng = np.random.default_rng(seed)
-    rows = rng.choice(geo.n_det, n_rows, replace=False)
-    planes = rng.integers(0, geo.shape[2], n_rows)
+    rows = []
+    for ly in np.unique(y):
+        idx = np.nonzero(y == ly)[0]
+        rows += [int(r) for r in rng.choice(idx, min(per_layer, idx.size), replace=False)]
+    return rows, rng.integers(0, geo.shape[2], len(rows))
+
+
+def sample(geo, nv, sub, per_layer, seed):
+    rows, planes = stratified(geo, per_layer, seed)
     X, Y, _ = geo.shape
-    flops = cross_det = cross_hole = 0.0
+    tot = np.zeros(5)
+    flops_all = 0.0
     n = 0
     for r, z in zip(rows, planes):
-        _, st = oracle.pe_count(geo, nv=nv, sub=sub, det_range=(int(r), int(r) + 1), z0=int(z), slab_z=1)
-        flops += st[0, 0]
-        cross_det += st[0, 1]
-        cross_hole += st[0, 2]
+        _, st = oracle.pe_count(geo, nv=nv, sub=sub, det_range=(r, r + 1), z0=int(z), slab_z=1, touched=True)
+        _, st_all = oracle.pe_count(geo, nv=nv, sub=sub, det_range=(r, r + 1), z0=int(z), slab_z=1)
+        tot += st[0]
+        flops_all += st_all[0, 0]
         n += X * Y
-    return {"flops_per_element": flops / n, "culled_crystals_per_element": cross_det / n,
-            "culled_holes_per_element": cross_hole / n, "elements_counted": n,
-            "sample": f"{n_rows} seeded detector rows x one voxel plane each (seed {seed})"}
+    return {"flops_per_element": flops_all / n, "flops_per_element_touched": tot[0] / n,
+            "culled_crystals_per_element": tot[1] / n, "culled_holes_per_element": tot[2] / n,
+            "touched_crystals_per_element": tot[3] / n, "touched_holes_per_element": tot[4] / n,
+            "elements_counted": n,
+            "sample": f"{per_layer} seeded detector rows per detector layer x one voxel plane each (seed {seed})"}
 
 
 def main():
@@ -47,8 +68,9 @@ def main():
     out["c1_pe_ref_sampling"] = sample(c1, 1, (8, 16, 8), 8, 1)
     out["c1_pe_nv4_sub222"] = sample(c1, 4, (2, 2, 2), 8, 2)
     c2 = params.config2()
-    out["c2_pe_nv3_sub222_detail"] = sample(c2, 3, (2, 2, 2), 24, 3)
+    out["c2_pe_nv3_sub222_detail"] = sample(c2, 3, (2, 2, 2), 6, 3)
     out["c2_pe_nv3_sub222"] = out["c2_pe_nv3_sub222_detail"]["flops_per_element"]
+    out["c2_pe_nv3_sub222_touched"] = out["c2_pe_nv3_sub222_detail"]["flops_per_element_touched"]
     with open(os.path.join(HERE, "flops_per_element.json"), "w") as f:
         json.dump(out, f, indent=1, sort_keys=True)
     print(json.dumps(out, indent=1))

@@ -1,0 +1,134 @@
+"""The scatter configurations at their own shapes.
+
+* Batched-variant geometry (six collimator variants from seed 0: 1-3 plates, 4x4 to
+  12x12 holes of 0.5-2.0 mm; 3 x 48x48 = 6912 crystals; 48^3 x 0.75 mm image): for
+  every variant, PE on the GPU, then crystal + collimator scatter of two seeded rows
+  (one in the front, one in the back layer) on the GPU, checked on a seeded voxel
+  sample against the FP64 oracle within 1e-5 (crystalScatterSysMatCuda scatter.cu:1077-1437,
+  collimatorScatterSysMatCuda :1544-1936).  The bitmap here would be 20.6 GB, so both
+  sides evaluate the GeometryRelationShip criterion inline.
+* Collimator-scatter geometry (2048 crystals): the GPU-built crystal bitmap
+  (537 MB, scatter.cu:1440-1540) bit-exact against the oracle's FP32 criterion on a
+  seeded sample of 384 pairs x all 2048 crystals; the collimator bitmap
+  (scatter.cu:1939-2009) bit-exact in full.
+* Surrogate geometry (4 x 64x64 = 16384 crystals, the 51.2 mm field of view at 3.2 mm
+  voxels): the surrogate trained on its own exact rows is judged on held-out rows
+  against the exact kernel (total crystal-scatter sum within 1 %) and beats the same
+  pipeline without the network."""
+from concurrent.futures import ThreadPoolExecutor
+
+import numpy as np
+import pytest
+
+import oracle
+from sysmat import params
+from sysmat import scatter as sc
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-5
+
+
+def rel_err(gpu, ref):
+    gpu, ref = np.asarray(gpu, np.float64), np.asarray(ref, np.float64)
+    rowmax = ref.max(axis=1, keepdims=True)
+    sig = (ref >= 1e-9 * rowmax) & (rowmax > 0)
+    return float(np.where(sig, np.abs(gpu - ref) / np.maximum(ref, 1e-300), 0).max()), int(sig.sum())
+
+
+def test_batched_variants_at_shape_sampled_rows(lib):
+    import torch
+    from sysmat.pe import PePlan
+    vs = params.config4_variants(seed=0)
+    assert len(vs) == 6 and all(v.n_det == 6912 and v.shape == (48, 48, 48) for v in vs)
+    N, V = vs[0].n_det, vs[0].n_vox
+    rng = np.random.default_rng(21)
+    vox = np.sort(rng.choice(V, 48, replace=False)).astype(np.int32)
+    jobs = []
+    for i, g in enumerate(vs):
+        rows = [int(rng.integers(0, 2304)), int(rng.integers(4608, 6912))]
+        print(f"variant {i}: {g.n_layers} plate(s), {g.n_holes} holes, rows {rows}", flush=True)
+        pe = torch.empty((N, V), dtype=torch.float32, device="cuda")
+        plan = PePlan(g, voxel_sub=1, sub=(2, 2, 2))
+        plan.launch(0, N, pe)
+        plan.sync()
+        plan.close()
+        sp = sc.ScatterPlan(g, bitmaps=False)
+        cry = torch.zeros((2, V), dtype=torch.float32, device="cuda")
+        col = torch.zeros((2, V), dtype=torch.float32, device="cuda")
+        sp.crystal_rows(pe, np.array(rows, np.int32), cry)
+        for j, r in enumerate(rows):
+            sp.collimator(r, 1, col[j:j + 1])
+        torch.cuda.synchronize()
+        sp.close()
+        vt = torch.from_numpy(vox).long().cuda()
+        jobs.append((g, rows, pe.index_select(1, vt).cpu().numpy(), cry.cpu().numpy()[:, vox],
+                     col.cpu().numpy()[:, vox]))
+        del pe
+        torch.cuda.empty_cache()
+
+    def ref_row(args):
+        g, r, pe_cols = args
+        c = oracle.scatter_crystal(g, pe_cols, None, det_range=(r, r + 1), precision="f64", vox=vox, threads=1)
+        h = oracle.scatter_collimator(g, None, det_range=(r, r + 1), precision="f64", vox=vox, threads=1)
+        return c, h
+    tasks = [(g, r, pe_cols) for g, rows, pe_cols, _, _ in jobs for r in rows]
+    with ThreadPoolExecutor(max_workers=12) as ex:
+        refs = list(ex.map(ref_row, tasks))
+    t = 0
+    for i, (g, rows, _, cry, col) in enumerate(jobs):
+        rc = np.concatenate([refs[t + j][0] for j in range(2)])
+        rh = np.concatenate([refs[t + j][1] for j in range(2)])
+        t += 2
+        ec, nc = rel_err(cry, rc)
+        eh, nh = rel_err(col, rh)
+        print(f"variant {i}: crystal {nc} elements max rel err {ec:.2e}; collimator {nh} elements {eh:.2e}",
+              flush=True)
+        assert nc > 0 and nh > 0
+        assert ec <= REL and eh <= REL
+
+
+def test_collimator_scatter_geometry_bitmaps_bit_exact(lib):
+    g = params.config3()
+    n = g.n_det
+    assert n == 2048
+    words = sc.bitmap_crystal(g)
+    assert words.size == (n * (n + 1) // 2 * n + 31) // 32  # 537 MB
+    rng = np.random.default_rng(5)
+    i = rng.integers(0, n - 1, 384)
+    j = np.array([int(rng.integers(a + 1, n)) for a in i])
+    want = oracle.crystal_bits_for_pairs(g, np.stack([i, j], 1))
+    pair = i.astype(np.int64) * n - i.astype(np.int64) * (i + 1) // 2 + j
+    bits = pair[:, None] * n + np.arange(n, dtype=np.int64)[None, :]
+    got = ((words[bits >> 5] >> (bits & 31).astype(np.uint32)) & 1).astype(np.uint8)
+    print(f"crystal bitmap: {int(want.sum())} of {want.size} sampled bits set", flush=True)
+    assert want.sum() > 0
+    assert np.array_equal(got, want)
+    hw = sc.bitmap_collimator(g)
+    assert np.array_equal(hw, oracle.bitmap_collimator(g))
+
+
+def test_surrogate_on_its_detector(lib):
+    import torch
+    from sysmat.pe import PePlan
+    from sysmat.surrogate import ScatterSurrogate, TrainConfig
+    g = params.config5()
+    g.img[0:3] = 16
+    g.img[3:6] = 3.2
+    N, V = g.n_det, g.n_vox
+    assert N == 16384
+    pe = torch.empty((N, V), dtype=torch.float32, device="cuda")
+    plan = PePlan(g, voxel_sub=1, sub=(2, 2, 2))
+    plan.launch(0, N, pe)
+    plan.sync()
+    plan.close()
+    s = ScatterSurrogate(g, use_bitmap=False)
+    cfg = TrainConfig(train_rows=128, judge_rows=32, vox_per_row=4096, members=2, steps=1500, seed=0)
+    t = s.train(pe, cfg)
+    print("train timings", t, flush=True)
+    j = s.judge(pe)
+    print("judge", j, flush=True)
+    assert abs(j["total_rel_err"]) <= 0.01
+    assert j["l1_rel_err"] < j["near_only_l1_rel_err"]
+    assert set(s.train_rows).isdisjoint(set(s.judge_rows))
+    s.close()

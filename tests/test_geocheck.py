@@ -96,3 +96,21 @@ def test_sysmat_reader_layout(tmp_path):
     assert vm.shape == (2, Z, Y, X, N)
     with pytest.raises(ValueError):
         matrix.open_sysmat(str(tmp_path / name), N + 1, (X, Y, Z), 2)
+
+
+def test_shipped_reference_params_fixture():
+    """PEGen_RayTracing_CircularHole/Params_*.dat (stored unchanged in tests/golden/ref_params):
+    80000/80000/100/100 floats, read by header counts; the holes' stored y range [0, 5] does
+    not coincide with the plate box at -2.5 .. 2.5 (PESysMatGen.cu:559-561 vs :650-651)."""
+    d = os.path.join(ROOT, "tests", "golden", "ref_params")
+    sizes = [os.path.getsize(os.path.join(d, f"Params_{n}.dat")) for n in ("Collimator", "Detector", "Image", "Physics")]
+    assert sizes == [320000, 320000, 400, 400]
+    g = params.Geometry.read(d)
+    assert (g.n_det, g.n_layers, g.n_holes, g.shape) == (5632, 1, 1865, (64, 64, 64))
+    params.check_tables(g.col, g.det)
+    h = g.holes
+    assert np.all(h[:, 1] == 0) and np.all(h[:, 2] == 5) and np.all(h[:, 4] == np.float32(0.8))
+    assert sorted(np.unique(g.crystals[:, 1]).tolist()) == [11.5, 14.5, 17.5, 20.0]
+    rep = geocheck.check(g)
+    assert rep.errors == [] or all("hole" in e for e in rep.errors)
+    assert any("hole" in s and "plate" in s for s in rep.errors + rep.warnings), rep.text()

@@ -357,3 +357,27 @@ def test_collimator_bitmap_against_brute_force():
                 seen += got
                 assert got == (d <= 0), (h, j, k)
     assert near <= 2 and 0 < seen < nh * n * n
+
+
+def test_voxel_sample_and_inline_criterion_equal_the_full_runs():
+    """The oracle's voxel-sample / bitmap-free forms (used at sizes whose bitmaps do not fit)
+    give exactly the full runs' numbers: same lists, same per-element arithmetic."""
+    g = params.tiny(seed=6, n_det_side=3, n_layers=2, image=(4, 3, 3), holes_side=2)
+    N, V = g.n_det, g.n_vox
+    pe = oracle.pe(g, sub=(2, 2, 2), precision="f64")
+    bc, bh = oracle.bitmap_crystal(g), oracle.bitmap_collimator(g)
+    vox = np.array([0, 7, 13, 35, 22], np.int32)
+    full_c = oracle.scatter_crystal(g, pe, bc, precision="f64")
+    full_h = oracle.scatter_collimator(g, bh, precision="f64")
+    assert full_c.max() > 0 and full_h.max() > 0
+    assert np.array_equal(oracle.scatter_crystal(g, pe, None, precision="f64"), full_c)
+    assert np.array_equal(oracle.scatter_collimator(g, None, precision="f64"), full_h)
+    assert np.array_equal(oracle.scatter_crystal(g, pe[:, vox], None, precision="f64", vox=vox), full_c[:, vox])
+    assert np.array_equal(oracle.scatter_collimator(g, None, precision="f64", vox=vox, det_range=(3, 9)),
+                          full_h[3:9][:, vox])
+    pairs = [(0, 5), (2, 17), (4, 9), (16, 17)]
+    got = oracle.crystal_bits_for_pairs(g, pairs)
+    for p, (i, j) in enumerate(pairs):
+        for k in range(N):
+            b = (i * N - i * (i + 1) // 2 + j) * N + k
+            assert got[p, k] == (int(bc[b >> 5]) >> (b & 31)) & 1
