@@ -10,15 +10,19 @@ geometry is resident before timing starts.
 
 Extra objects on the same JSON line:
   roofline      the tracer kernel against the FP32 vector peak (algorithmic
-                FLOPs per element from the op-counting restatement), HBM
-                traffic per launch from the committed rocprofv3 summary;
+                FLOPs per element from the op-counting restatement, counting
+                the attenuators that touch the element), HBM traffic per
+                launch and executed FLOP/s from the committed rocprofv3 summary;
   cpu_baseline  the FP32 restatement of the reference kernel (oracle/, g++
                 -O3 OpenMP) on this host's cores, bounded sample;
   pe_scatter    PE + primary Compton scatter, plain exact kernels vs the
                 learned surrogate (exact near field + MLP far field +
                 exact fallback), on the collimator-scatter geometry (fully
-                measured) and on the surrogate geometry (N = 16384; exact
-                crystal scatter timed on a row sample and projected).
+                measured) and on the surrogate geometry (N = 16384, 128^3 x
+                0.4 mm image; PE, collimator scatter and training over every
+                row, the exact crystal scatter and the surrogate apply timed on
+                the same seeded row sample and projected), with the MLP
+                kernel's MFMA roofline.
 """
 import argparse
 import json
@@ -34,6 +38,7 @@ for p in (ROOT, PKG):
         sys.path.insert(0, p)
 
 FP32_PEAK_TFLOPS = 157.3  # MI355X published FP32 vector peak (dense)
+BF16_MFMA_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 HBM_PEAK_GBS = 8000.0
 METRIC = "sysmat elements/s (PE, PE+scatter) plain vs surrogate @1% err, 1 MI355X"
 
@@ -265,13 +270,16 @@ def scatter_c4(dev, log):
     return res
 
 
-def scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_block=2048, tag="C5r"):
+MLP_FLOP_PER_ELEMENT = 2 * (12 * 256 + 3 * 256 * 256 + 256)  # 399,872 MFMA FLOP per element per member
+
+
+def scatter_c5(dev, log, image=128, voxel=0.4, sample_rows=64, apply_all=False, apply_block=2048, tag="C5"):
     """Surrogate geometry (16384 detectors in 4 layers; image ``image``^3 voxels of ``voxel``
-    mm): PE, the collimator scatter, surrogate training and the surrogate apply (every row,
-    in blocks of ``apply_block`` rows) measured in full; the exact crystal scatter timed
-    on a seeded row sample and projected linearly to all rows.  The PE matrix stays
-    resident (the surrogate reads the rows of the scatter sites); the collimator scatter
-    and the surrogate output are produced block by block into reused buffers."""
+    mm): PE and the collimator scatter of every row, surrogate training on its own exact rows,
+    the judge rows; the exact crystal scatter timed on a seeded row sample and projected
+    linearly to all rows; the surrogate apply (near field + MLP + exact fallback) on every
+    row (``apply_all``) or on the same sampled rows, projected likewise.  The PE matrix stays
+    resident (the scatter reads the rows of every scatter site)."""
     import numpy as np
     import torch
     from sysmat import params
@@ -285,7 +293,7 @@ def scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_block=2048,
     pe, t_pe = pe_full(g, 1, (2, 2, 2), dev, log=log if long_run else None)
     log(f"{tag} PE {t_pe:.2f}s")
     plan = ScatterPlan(g, bitmaps=False)
-    blk = torch.empty((apply_block, V), dtype=torch.float32, device=dev)
+    blk = torch.empty((min(apply_block, N), V), dtype=torch.float32, device=dev)
     t0 = sync_t()
     for b in range(0, N, apply_block):
         plan.collimator(b, min(apply_block, N - b), blk)
@@ -299,30 +307,79 @@ def scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_block=2048,
     t0 = sync_t()
     plan.crystal_rows(pe, rows, ex)
     t_ex = sync_t() - t0
-    del ex
     log(f"{tag} exact crystal {sample_rows} rows {t_ex:.2f}s, collimator all rows {t_col:.2f}s")
     sur = ScatterSurrogate(g, use_bitmap=False)
     t0 = sync_t()
     tr = sur.train(pe, TrainConfig(seed=0))
     t_train = sync_t() - t0
-    log(f"{tag} surrogate trained {t_train:.2f}s")
+    log(f"{tag} surrogate trained {t_train:.2f}s ({tr['exact_rows_s']:.2f}s exact training rows, "
+        f"{tr['train_s']:.2f}s optimisation)")
     judge = sur.judge(pe)
     log(f"{tag} judge rows done")
-    # surrogate apply (near field + MLP + exact fallback) over every row, in row blocks
+    # surrogate apply: near field, MLP kernel and exact fallback timed separately (HIP events on the
+    # launch stream)
+    phase = np.zeros(3)
     n_fb = 0
+    applied_rows = 0
+
+    def run(b, cnt, out):
+        nonlocal n_fb, applied_rows
+        ev = [torch.cuda.Event(enable_timing=True) for _ in range(4)]
+        n_fb += sur.apply(pe, b, cnt, out, events=ev)
+        torch.cuda.synchronize()
+        phase[:] += [ev[0].elapsed_time(ev[1]), ev[1].elapsed_time(ev[2]), ev[2].elapsed_time(ev[3])]
+        applied_rows += cnt
+
+    sur_out = torch.empty((sample_rows, V), dtype=torch.float32, device=dev)
     t0 = sync_t()
-    for b in range(0, N, apply_block):
-        n_fb += sur.apply(pe, b, min(apply_block, N - b), blk)
-        if long_run:
-            log(f"  surrogate rows {b + min(apply_block, N - b)}/{N} {sync_t() - t0:.1f}s")
+    if apply_all:
+        for b in range(0, N, apply_block):
+            run(b, min(apply_block, N - b), blk)
+            if long_run:
+                log(f"  surrogate rows {b + min(apply_block, N - b)}/{N} {sync_t() - t0:.1f}s")
+    else:
+        for i, r in enumerate(rows):
+            run(int(r), 1, sur_out[i:i + 1])
     t_ap = sync_t() - t0
-    log(f"{tag} surrogate apply all rows {t_ap:.2f}s (train {t_train:.2f}s)")
-    per_row_exact = t_ex / sample_rows
-    t_cry_full = per_row_exact * N
-    t_sur_full = t_ap + t_train
+    scale = N / applied_rows
+    log(f"{tag} surrogate apply {applied_rows} rows {t_ap:.2f}s (near {phase[0] / 1e3:.2f}s, MLP "
+        f"{phase[1] / 1e3:.2f}s, fallback {phase[2] / 1e3:.2f}s; train {t_train:.2f}s)")
+    t_cry_full = t_ex / sample_rows * N
+    t_sur_full = t_ap * scale + t_train
     elems = N * V
     plain_t = t_pe + t_col + t_cry_full
     sur_t = t_pe + t_col + t_sur_full
+    mlp_s = phase[1] / 1e3
+    mfma = applied_rows * V * len(sur.members) * MLP_FLOP_PER_ELEMENT / mlp_s / 1e12
+    res = {
+        "workload": f"surrogate geometry: 4 x 64x64 crystals (16384 det), 1 W plate 6x6 holes, image "
+                    f"{image}^3 x {voxel} mm, PE sampling (1, 2x2x2)",
+        "elements": elems,
+        "projection": (f"PE, collimator scatter and training over every row; exact crystal scatter timed on "
+                       f"{sample_rows} seeded rows and scaled to {N} rows; surrogate apply "
+                       + (f"over every row (blocks of {apply_block})" if apply_all else
+                          f"timed on the same {sample_rows} rows and scaled to {N} rows")),
+        "plain": {"elements_per_s": round(elems / plain_t, 1), "pe_s": round(t_pe, 3),
+                  "collimator_scatter_s": round(t_col, 3), "crystal_scatter_s_projected": round(t_cry_full, 2)},
+        "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
+                      "train_exact_rows_s": round(tr["exact_rows_s"], 3), "train_optimise_s": round(tr["train_s"], 3),
+                      "apply_s": round(t_ap * scale, 3),
+                      "apply_phase_s": {"near_field": round(phase[0] / 1e3 * scale, 3),
+                                        "mlp_kernel": round(mlp_s * scale, 3),
+                                        "exact_fallback": round(phase[2] / 1e3 * scale, 3)},
+                      "fallback_frac": round(n_fb / (applied_rows * sur.n_tiles), 4),
+                      "train_pairs": tr["pairs"]},
+        "crystal_scatter_speedup_incl_training": round(t_cry_full / t_sur_full, 2),
+        "pe_plus_scatter_speedup": round(plain_t / sur_t, 2),
+        "judge_heldout": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in judge.items()},
+        "mlp_roofline": {"kernel": "surrogate_apply_kernel", "bound": "mfma", "achieved": round(mfma, 2),
+                         "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s",
+                         "frac": round(mfma / BF16_MFMA_PEAK_TFLOPS, 4),
+                         "flops_per_element": MLP_FLOP_PER_ELEMENT * len(sur.members),
+                         "basis": f"{len(sur.members)} members x 2 (12x256 + 3 x 256x256 + 256) MFMA FLOP per "
+                                  f"element, {applied_rows} rows x {V} voxels / the kernel's HIP-event time",
+                         "kernel_s": round(mlp_s, 4)},
+    }
     # total-sum error on the judge rows for the whole system matrix (PE + collimator + crystal)
     jr = sur.judge_rows
     jpe = pe[torch.as_tensor(jr, device=dev).long()].double().sum().item()
@@ -330,33 +387,13 @@ def scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_block=2048,
     for r in jr:
         plan.collimator(int(r), 1, blk)
         jcol += blk[0].double().sum().item()
-    res = {
-        "workload": f"surrogate geometry: 4 x 64x64 crystals (16384 det), 1 W plate 6x6 holes, image "
-                    f"{image}^3 x {voxel} mm, PE sampling (1, 2x2x2)",
-        "elements": elems,
-        "projection": f"exact crystal scatter timed on {sample_rows} seeded rows with a contribution and "
-                      f"scaled to {N} rows; PE, collimator scatter, training and the surrogate apply "
-                      f"(all {N} rows, blocks of {apply_block}) measured in full",
-        "plain": {"elements_per_s": round(elems / plain_t, 1), "pe_s": round(t_pe, 3),
-                  "collimator_scatter_s": round(t_col, 3), "crystal_scatter_s_projected": round(t_cry_full, 2)},
-        "surrogate": {"elements_per_s": round(elems / sur_t, 1), "train_s": round(t_train, 3),
-                      "apply_s": round(t_ap, 3),
-                      "fallback_frac": round(n_fb / (N * sur.n_tiles), 4),
-                      "train_pairs": tr["pairs"]},
-        "crystal_scatter_speedup_incl_training": round(t_cry_full / t_sur_full, 2),
-        "pe_plus_scatter_speedup": round(plain_t / sur_t, 2),
-        "judge_heldout": {k: (round(v, 6) if isinstance(v, float) else v) for k, v in judge.items()},
-        "system_matrix_total_sum_rel_err_judge_rows": None,
-    }
-    cry_tot = judge["total_rel_err"]
-    # crystal-scatter sum of the judge rows from the exact kernel
     jex = torch.empty((len(jr), V), dtype=torch.float32, device=dev)
     sur.exact_rows(pe, jr, jex)
     jcry = jex.double().sum().item()
-    res["system_matrix_total_sum_rel_err_judge_rows"] = round(cry_tot * jcry / (jpe + jcol + jcry), 7)
+    res["system_matrix_total_sum_rel_err_judge_rows"] = round(judge["total_rel_err"] * jcry / (jpe + jcol + jcry), 7)
     plan.close()
     sur.close()
-    del pe, blk, jex
+    del pe, blk, jex, ex, sur_out
     torch.cuda.empty_cache()
     return res
 
@@ -370,8 +407,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scatter", action="store_true", help="skip the PE+scatter legs")
     ap.add_argument("--c4", action="store_true", help="also run the batched collimator-variant leg (minutes)")
-    ap.add_argument("--c5-full", action="store_true",
-                    help="also run the surrogate leg on the full 128^3 x 0.4 mm image (about 12 minutes, 150 GB HBM)")
+    ap.add_argument("--c5-reduced", action="store_true",
+                    help="surrogate leg on a 32^3 x 1.6 mm image (every row applied) instead of the 128^3 x 0.4 mm one")
     ap.add_argument("--cpu-rows", type=int, default=0,
                     help="CPU baseline rows per detector layer (0: host threads / 4, threads <= 16)")
     ap.add_argument("--calib-copy", action="store_true",
@@ -496,7 +533,11 @@ def main():
         except Exception as e:  # reported, not hidden
             pe_scatter["c3"] = {"error": repr(e)}
         try:
-            pe_scatter["c5"] = scatter_c5(dev, log)
+            if args.c5_reduced:
+                pe_scatter["c5"] = scatter_c5(dev, log, image=32, voxel=1.6, sample_rows=512, apply_all=True,
+                                              tag="C5r")
+            else:
+                pe_scatter["c5"] = scatter_c5(dev, log)
         except Exception as e:
             pe_scatter["c5"] = {"error": repr(e)}
 
@@ -511,18 +552,6 @@ def main():
             pe_scatter["c4"] = scatter_c4(dev, log)
         except Exception as e:
             pe_scatter["c4"] = {"error": repr(e)}
-    if args.c5_full:
-        if pe_scatter is None:
-            plan.close()
-            del out
-            torch.cuda.empty_cache()
-            pe_scatter = {}
-        try:
-            pe_scatter["c5_full"] = scatter_c5(dev, log, image=128, voxel=0.4, sample_rows=64, apply_block=1024,
-                                               tag="C5")
-        except Exception as e:
-            pe_scatter["c5_full"] = {"error": repr(e)}
-
     cpu = None
     if not args.no_cpu:
         rng = np.random.default_rng(0)

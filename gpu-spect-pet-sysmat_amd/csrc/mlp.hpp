@@ -35,9 +35,14 @@ constexpr long long kPkW5 = kPkBias + 4 * 1024;    // 256 floats
 constexpr long long kPkB5 = kPkW5 + 1024;          // 1 float (+pad)
 constexpr long long kMlpPacked = kPkB5 + 256;
 
-__device__ __forceinline__ float silu_f(float z) { return z / (1.0f + __expf(-z)); }
+// SiLU and its derivative with the logistic through v_rcp_f32 (1 ulp) instead of an
+// IEEE division (~10 instructions): the activations are rounded to bf16 next, so the
+// ulp does not show, and the epilogues, not the MFMAs, set the inference kernel's
+// VALU time.
+__device__ __forceinline__ float sigmoid_f(float z) { return __builtin_amdgcn_rcpf(1.0f + __expf(-z)); }
+__device__ __forceinline__ float silu_f(float z) { return z * sigmoid_f(z); }
 __device__ __forceinline__ float dsilu_f(float z) {
-    const float s = 1.0f / (1.0f + __expf(-z));
+    const float s = sigmoid_f(z);
     return s * (1.0f + z * (1.0f - s));
 }
 

@@ -225,24 +225,38 @@ class ScatterSurrogate:
 
     # ---- inference ---------------------------------------------------------
     def apply(self, pe, det_begin: int, det_count: int, dst, row_stride: int = 0, fallback: bool = True,
-              n_members: Optional[int] = None, tol: Optional[float] = None, stream=None) -> int:
-        """Surrogate crystal scatter of rows [det_begin, +det_count) into dst; returns fallback tiles."""
+              n_members: Optional[int] = None, tol: Optional[float] = None, stream=None, events=None) -> int:
+        """Surrogate crystal scatter of rows [det_begin, +det_count) into dst; returns fallback tiles.
+
+        events (optional list): four torch.cuda.Event(enable_timing=True) recorded on the launch
+        stream before the near field, before the MLP kernel, before the fallback and at the end."""
         import torch
         if self.packed is None:
             raise RuntimeError("surrogate not trained")
         nm = len(self.members) if n_members is None else n_members
+        ts = None
+        if events is not None:
+            ts = stream if stream is not None else torch.cuda.current_stream()
+            events[0].record(ts)
         self.near(pe, det_begin, det_count, dst, row_stride, stream)
         n_tiles = self.n_tiles
         flags = torch.zeros(det_count * n_tiles, dtype=torch.uint8, device=pe.device) if fallback else None
         floor_abs = self.cfg.floor_rel * self.norm["s_scale"]
         tol = self.cfg.tol if tol is None else tol
+        if ts is not None:
+            events[1].record(ts)
         check(self.L.sm_surrogate_apply(self._h, ptr(self.packed), nm, ptr(pe), det_begin, det_count, ptr(dst),
                                         row_stride, ctypes.c_float(tol), ctypes.c_float(floor_abs),
                                         None if flags is None else ptr(flags), _sh(stream)))
-        if not fallback:
-            return 0
-        return int(check(self.L.sm_surrogate_fallback(self._h, ptr(pe), det_begin, det_count, ptr(flags), ptr(dst),
-                                                      row_stride, _sh(stream))))
+        if ts is not None:
+            events[2].record(ts)
+        n_fb = 0
+        if fallback:
+            n_fb = int(check(self.L.sm_surrogate_fallback(self._h, ptr(pe), det_begin, det_count, ptr(flags),
+                                                          ptr(dst), row_stride, _sh(stream))))
+        if ts is not None:
+            events[3].record(ts)
+        return n_fb
 
     def judge(self, pe, rows: Optional[Sequence[int]] = None, fallback: bool = True,
               tol: Optional[float] = None) -> dict:
