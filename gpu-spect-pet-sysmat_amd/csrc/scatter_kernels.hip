@@ -9,8 +9,12 @@
 //                             collimator model (sites = cells, weights from
 //                             cell_weight_kernel)
 //
-// The bitmap kernels reproduce the reference bits exactly: IEEE FP32 division
-// and square root, no FMA contraction, the reference's operation order.  The
+// The bitmap kernels evaluate the reference's criterion in the reference's
+// operation order with IEEE FP32 division and square root and no FMA
+// contraction, and match the FP32 restatement of it bit for bit.  nvcc contracts
+// the reference's sums of products into FMAs by default, so a criterion value
+// that ties its bound within an ulp may round the other way there: bit parity
+// with files the reference's own build writes is unpinned.  The
 // scatter kernels evaluate everything that decides a branch (cone angle,
 // scattered energy and the energy-window cut, the theta range and the
 // Riemann step count of the KN weight, the "line through the detector" test)
