@@ -22,7 +22,9 @@ Extra objects on the same JSON line:
                 0.4 mm image; PE, collimator scatter and training over every
                 row, the exact crystal scatter and the surrogate apply timed on
                 the same seeded row sample and projected), with the MLP
-                kernel's MFMA roofline.
+                kernel's MFMA roofline; and the batched-variant geometry (6
+                collimator variants, crystal + collimator scatter of all of them
+                in one captured hipGraph, per-variant rates and FP64 errors).
 """
 import argparse
 import json
@@ -216,14 +218,20 @@ def scatter_c3(dev, log):
     return res
 
 
-def scatter_c4(dev, log):
+def scatter_c4(dev, log, sample_rows=256, check_rows=2, check_vox=48):
     """Batched variants: 6 collimator variants (seed 0) sharing a 3 x 48x48 detector and a
-    48^3 x 0.75 mm image; per variant PE (1, 2x2x2), then crystal + collimator scatter of
-    every variant from one captured hipGraph (and each variant in a graph of its own)."""
+    48^3 x 0.75 mm image.  Per variant the PE matrix (1, 2x2x2); then crystal + collimator
+    scatter of every variant and every row in ONE captured hipGraph (timed: capture + run);
+    per-variant scatter rates from a seeded row sample of each variant's plan (eager,
+    projected to all rows); per-variant error against the FP64 oracle on sampled rows x
+    voxels of the graph's own output."""
+    from concurrent.futures import ThreadPoolExecutor
+    import numpy as np
     import torch
+    import oracle
     from sysmat import params
     from sysmat.pe import PePlan
-    from sysmat.scatter import ScatterBatch
+    from sysmat.scatter import ScatterBatch, ScatterPlan
     vs = params.config4_variants(seed=0)
     N, V = vs[0].n_det, vs[0].n_vox
     pes, pe_t = [], []
@@ -238,33 +246,62 @@ def scatter_c4(dev, log):
         pes.append(pe)
     log(f"C4 PE per variant {[round(t, 2) for t in pe_t]}")
     outs = [torch.empty((N, V), dtype=torch.float32, device=dev) for _ in vs]
-    per = []
-    for i, g in enumerate(vs):
-        b = ScatterBatch([g])
-        t0 = sync_t()
-        b.run([pes[i]], [outs[i]], use_graph=True)
-        per.append(sync_t() - t0)
-        b.close()
-        log(f"C4 variant {i}: scatter {per[-1]:.2f}s")
     b = ScatterBatch(vs)
     t0 = sync_t()
     b.run(pes, outs, use_graph=True)
     t_all = sync_t() - t0
     nodes = b.graph_nodes
-    log(f"C4 all variants in one graph ({nodes} nodes, capture + run): {t_all:.2f}s")
-    t0 = sync_t()
-    b.run(pes, outs, use_graph=True)  # replay of the captured graph
-    t_replay = sync_t() - t0
     b.close()
-    log(f"C4 graph replay: {t_replay:.2f}s")
+    log(f"C4 all variants in one captured graph ({nodes} nodes): {t_all:.2f}s")
+    rng = np.random.default_rng(7)
+    rows = np.sort(rng.choice(N, sample_rows, replace=False)).astype(np.int32)
+    vox = np.sort(rng.choice(V, check_vox, replace=False)).astype(np.int32)
+    vt = torch.from_numpy(vox).long().to(dev)
+    per, checks = [], []
+    for i, g in enumerate(vs):
+        sp = ScatterPlan(g, bitmaps=False)
+        cry = torch.empty((sample_rows, V), dtype=torch.float32, device=dev)
+        col = torch.empty((sample_rows, V), dtype=torch.float32, device=dev)
+        t0 = sync_t()
+        sp.crystal_rows(pes[i], rows, cry)
+        for j, r in enumerate(rows):
+            sp.collimator(int(r), 1, col[j:j + 1])
+        t = sync_t() - t0
+        sp.close()
+        per.append(t * N / sample_rows)
+        crow = [int(rows[0]), int(rows[-1])][:check_rows]
+        checks.append((g, crow, pes[i].index_select(1, vt).cpu().numpy(),
+                       outs[i][torch.as_tensor(crow, device=dev).long()].index_select(1, vt).cpu().numpy()))
+        del cry, col
+    log(f"C4 per-variant scatter (projected from {sample_rows} rows): {[round(t, 1) for t in per]}")
+
+    def ref(args):
+        g, r, pe_cols = args
+        c = oracle.scatter_crystal(g, pe_cols, None, det_range=(r, r + 1), precision="f64", vox=vox, threads=1)
+        h = oracle.scatter_collimator(g, None, det_range=(r, r + 1), precision="f64", vox=vox, threads=1)
+        return (c + h.astype(np.float64))[0]
+    with ThreadPoolExecutor(max_workers=min(16, os.cpu_count() or 4)) as ex:
+        refs = list(ex.map(ref, [(g, r, pc) for g, crow, pc, _ in checks for r in crow]))
+    errs, k = [], 0
+    for g, crow, _, got in checks:
+        want = np.stack(refs[k:k + len(crow)]).astype(np.float64)
+        k += len(crow)
+        rowmax = want.max(axis=1, keepdims=True)
+        sig = (want >= 1e-9 * rowmax) & (rowmax > 0)
+        errs.append(float(np.where(sig, np.abs(got - want) / np.maximum(want, 1e-300), 0).max()))
+    log(f"C4 per-variant max rel err vs FP64 oracle {[f'{e:.1e}' for e in errs]}")
     elems = len(vs) * N * V
     res = {"workload": "6 collimator variants (1-3 plates, 4x4..12x12 holes), shared 3 x 48x48 detector (6912 det), "
                        "48^3 x 0.75 mm image; PE (1, 2x2x2) + crystal + collimator scatter",
            "elements": elems, "pe_s": [round(t, 3) for t in pe_t],
-           "scatter_s_per_variant_graph": [round(t, 3) for t in per],
-           "scatter_s_all_one_graph": round(t_all, 3), "scatter_s_replay": round(t_replay, 3),
-           "graph_nodes": nodes,
-           "pe_plus_scatter_elements_per_s": round(elems / (sum(pe_t) + t_replay), 1)}
+           "scatter_s_all_one_graph": round(t_all, 3), "graph_nodes": nodes,
+           "pe_plus_scatter_elements_per_s": round(elems / (sum(pe_t) + t_all), 1),
+           "per_variant": [{"plates": g.n_layers, "holes": g.n_holes, "pe_s": round(pe_t[i], 3),
+                            "scatter_s_projected": round(per[i], 2),
+                            "pe_plus_scatter_elements_per_s": round(N * V / (pe_t[i] + per[i]), 1),
+                            "max_rel_err_vs_fp64": float(f"{errs[i]:.3g}")} for i, g in enumerate(vs)],
+           "per_variant_basis": f"eager plan on {sample_rows} seeded rows, projected; FP64 error on "
+                                f"{check_rows} rows x {check_vox} voxels of the graph output"}
     del pes, outs
     torch.cuda.empty_cache()
     return res
@@ -406,7 +443,7 @@ def main():
     ap.add_argument("--block", type=int, default=1024, help="detector rows per step")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-scatter", action="store_true", help="skip the PE+scatter legs")
-    ap.add_argument("--c4", action="store_true", help="also run the batched collimator-variant leg (minutes)")
+    ap.add_argument("--no-c4", action="store_true", help="skip the batched collimator-variant leg (~2 minutes)")
     ap.add_argument("--c5-reduced", action="store_true",
                     help="surrogate leg on a 32^3 x 1.6 mm image (every row applied) instead of the 128^3 x 0.4 mm one")
     ap.add_argument("--cpu-rows", type=int, default=0,
@@ -542,7 +579,7 @@ def main():
             pe_scatter["c5"] = {"error": repr(e)}
 
 
-    if args.c4:
+    if not args.no_c4:
         if pe_scatter is None:
             plan.close()
             del out
@@ -552,6 +589,7 @@ def main():
             pe_scatter["c4"] = scatter_c4(dev, log)
         except Exception as e:
             pe_scatter["c4"] = {"error": repr(e)}
+
     cpu = None
     if not args.no_cpu:
         rng = np.random.default_rng(0)
@@ -588,7 +626,7 @@ def main():
     if cpu:
         rec["speedup_vs_cpu"] = round(value / cpu["value"], 1)
     print(json.dumps(rec))
-    if args.no_scatter and not args.c4:
+    if args.no_scatter and args.no_c4:
         plan.close()
 
 
