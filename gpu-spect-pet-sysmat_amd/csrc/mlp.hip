@@ -179,9 +179,11 @@ __global__ void __launch_bounds__(256) head_bwd_kernel(const float* __restrict__
     const float wk = P[kOffW5 + k];
     float gw = 0.0f, gb = 0.0f;
     const long long b0 = (long long)blockIdx.x * R;
-    for (int i = 0; i < R; ++i) {
+    const int nr = (int)std::min<long long>(R, B - b0);
+    // unrolled so the row loads are in flight together (the sums keep their order)
+#pragma unroll 8
+    for (int i = 0; i < nr; ++i) {
         const long long b = b0 + i;
-        if (b >= B) break;
         const float z = Z4[b * kMlpH + k], d = dy[b];
         const float sg = 1.0f / (1.0f + __expf(-z));
         gw += d * (z * sg);
@@ -198,11 +200,9 @@ __global__ void __launch_bounds__(256) colsum_kernel(const bf16* __restrict__ D,
     const int k = threadIdx.x;
     float s = 0.0f;
     const long long b0 = (long long)blockIdx.x * R;
-    for (int i = 0; i < R; ++i) {
-        const long long b = b0 + i;
-        if (b >= B) break;
-        s += (float)D[b * kMlpH + k];
-    }
+    const int nr = (int)std::min<long long>(R, B - b0);
+#pragma unroll 16
+    for (int i = 0; i < nr; ++i) s += (float)D[(b0 + i) * kMlpH + k];
     part[(long long)blockIdx.x * kMlpH + k] = s;
 }
 
@@ -212,6 +212,7 @@ __global__ void reduce_parts_kernel(const float* __restrict__ part, int S, long 
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     float s = 0.0f;
+#pragma unroll 16
     for (int q = 0; q < S; ++q) s += part[(long long)q * stride + i];
     out[i] = s;
 }
@@ -223,6 +224,7 @@ __global__ void reduce_w_kernel(const float* __restrict__ part, int S, int M, lo
     if (i >= (long long)M * ncols) return;
     const long long m = i / ncols, c = i % ncols;
     float s = 0.0f;
+#pragma unroll 16
     for (int q = 0; q < S; ++q) s += part[(long long)q * M * ldc + m * ldc + c];
     out[m * ld_out + c] = s;
 }
@@ -361,8 +363,11 @@ struct Ws {
 
 long long align256(long long x) { return (x + 255) / 256 * 256; }
 
+// K splits of the weight-gradient GEMMs (K = batch): 128 rows per split, so a
+// 256 x 256 gradient at batch 4096 runs as 16 tiles x 32 splits = 512 workgroups
+// (two per CU) instead of 128, and each walks 4 instead of 16 k-steps.
 int wgrad_splits(long long B) {
-    const long long s = std::max(1LL, std::min(64LL, B / 512));
+    const long long s = std::max(1LL, std::min(64LL, B / 128));
     return (int)s;
 }
 
