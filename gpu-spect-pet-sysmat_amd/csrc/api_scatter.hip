@@ -852,6 +852,7 @@ struct SmSurrogate {
     DeviceBuf<int> keep_cnt;   // [N]
     DeviceBuf<SgRow> srow;     // [N]
     DeviceBuf<float> ftab;     // [N][kSgTab + 1] angular factor of far sites
+    DeviceBuf<float4> ring;    // [N][n_keep][2] compact ranked sites for the ring prior
     SgParams q{};
     int n_near = 0, n_ring = 0;
 };
@@ -969,6 +970,8 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     } else {
         HIP_TRY(hipMemset(S->keep_cnt.ptr, 0, sizeof(int) * n));
     }
+    if ((rc = S->ring.alloc((size_t)n * n_keep * 2)) < 0) return fail(rc);
+    HIP_TRY(launch_surrogate_ring_table(S->keep.ptr, S->keep_cnt.ptr, n, n_keep, S->ring.ptr, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     // per-row feature constants
     float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
@@ -1025,6 +1028,7 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     q.phi = 0.0f;
     q.n_tiles = (long long)((b.X + 3) / 4) * ((b.Y + 3) / 4) * ((b.Z + 3) / 4);
     q.ftab = S->ftab.ptr;
+    q.ring = S->ring.ptr;
     *out = S;
     return SM_OK;
 }

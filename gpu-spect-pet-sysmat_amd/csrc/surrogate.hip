@@ -9,17 +9,69 @@
 
 namespace sysmat {
 
-// Features of element (row r, voxel v); keep/cnt: the row's ranked sites.
-__device__ inline void sg_features(const SgParams& q, const SgRow& R, const ScPair* __restrict__ keep, int cnt,
-                                   int r, long long v, const float* __restrict__ pe, float s_near, float* f) {
+// Image-frame position of voxel v (rotated like the scatter kernels).
+__device__ inline void sg_position(const SgParams& q, long long v, float& xI, float& yI, float& zI) {
     const long long XY = (long long)q.X * q.Y;
     const int iz = (int)(v / XY);
     const int rem = (int)(v % XY);
     const int iy = rem / q.X, ix = rem % q.X;
     const float x0 = ((float)ix - (float)q.X * 0.5f + 0.5f) * q.wx + q.sx;
     const float y0 = ((float)iy - (float)q.Y * 0.5f + 0.5f) * q.wy + q.sy;
-    const float zI = ((float)iz - (float)q.Z * 0.5f + 0.5f) * q.wz + q.sz;
-    const float xI = x0 * q.rc - y0 * q.rs, yI = x0 * q.rs + y0 * q.rc;
+    zI = ((float)iz - (float)q.Z * 0.5f + 0.5f) * q.wz + q.sz;
+    xI = x0 * q.rc - y0 * q.rs;
+    yI = x0 * q.rs + y0 * q.rc;
+}
+
+// Ring prior over sites j = j0, j0 + 2, ... < cnt of the row's ranked list (rt:
+// the row's two float4 per site): each site's weight re-evaluated at this
+// voxel's scatter angle.  Unrolled so several PE gathers are in flight; a
+// zero-weight site adds exactly zero (its load is skipped).  The prior is the
+// even-site sum plus the odd-site sum, so the fused kernel can give the two
+// halves to the two half-waves.
+constexpr int kRingUnroll = 4;
+__device__ inline float sg_ring(const SgParams& q, const float4* __restrict__ rt, int j0, int cnt,
+                                const float* __restrict__ tab, const float* __restrict__ pe, long long v, float xI,
+                                float yI, float zI) {
+    float ring = 0.0f;
+#pragma unroll kRingUnroll
+    for (int j = j0; j < cnt; j += 2) {
+        const float4 a = rt[2 * j], b = rt[2 * j + 1];
+        const float wg = a.w;
+        const float ax = a.x - xI, ay = a.y - yI, az = a.z - zI;
+        const float c = (ax * b.x + ay * b.y + az * b.z) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
+        const float pv = wg != 0.0f ? pe[(long long)__float_as_int(b.w) * q.V + v] : 0.0f;
+        ring += wg != 0.0f ? pv * wg * sg_angular(tab, c) : 0.0f;
+    }
+    return ring;
+}
+
+__global__ void surrogate_ring_table_kernel(const ScPair* __restrict__ keep, const int* __restrict__ keep_cnt,
+                                            int n_det, int n_keep, float4* __restrict__ ring) {
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)n_det * n_keep) return;
+    const int r = (int)(i / n_keep), j = (int)(i % n_keep);
+    float4 a = {0.f, 0.f, 0.f, 0.f}, b = {0.f, 0.f, 0.f, 0.f};
+    if (j < keep_cnt[r]) {
+        const ScPair& S = keep[i];
+        a = float4{(float)S.sx, (float)S.sy, (float)S.sz, S.hy1};
+        b = float4{(float)S.bx, (float)S.by, (float)S.bz, __int_as_float(S.site)};
+    }
+    ring[2 * i] = a;
+    ring[2 * i + 1] = b;
+}
+
+hipError_t launch_surrogate_ring_table(const ScPair* keep, const int* keep_cnt, int n_det, int n_keep, float4* ring,
+                                      hipStream_t s) {
+    const long long n = (long long)n_det * n_keep;
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(surrogate_ring_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keep, keep_cnt,
+                       n_det, n_keep, ring);
+    return hipGetLastError();
+}
+
+// Features of element (row r, voxel v) from its position and ring prior.
+__device__ inline void sg_finish(const SgParams& q, const SgRow& R, int r, long long v, const float* __restrict__ pe,
+                                 float s_near, float ring, float xI, float yI, float zI, float* f) {
     const float inv_rho = 1.0f / q.irho;
     f[0] = (xI - q.icx) * inv_rho;
     f[1] = (yI - q.icy) * inv_rho;
@@ -27,17 +79,6 @@ __device__ inline void sg_features(const SgParams& q, const SgRow& R, const ScPa
     f[3] = R.f3;
     f[4] = R.f4;
     f[5] = R.f5;
-    // ring prior: sites n_near.. with their weight re-evaluated at this voxel's scatter angle
-    float ring = 0.0f;
-    const float* tab = q.ftab + (long long)r * (kSgTab + 1);
-    for (int j = q.n_near; j < cnt; ++j) {
-        const ScPair& S = keep[j];
-        const float wg = S.hy1;
-        if (wg == 0.0f) continue;
-        const float ax = (float)S.sx - xI, ay = (float)S.sy - yI, az = (float)S.sz - zI;
-        const float c = (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
-        ring += pe[(long long)S.site * q.V + v] * wg * sg_angular(tab, c);
-    }
     const float inv_s = 1.0f / q.s_scale;
     f[6] = 0.1f * __logf(s_near * inv_s + 1e-6f);
     f[7] = 0.1f * __logf(ring * inv_s + 1e-6f);
@@ -61,8 +102,15 @@ __global__ void __launch_bounds__(256) surrogate_features_kernel(SgParams q, con
     const int rr = (int)(i / n_vox), j = (int)(i % n_vox);
     const int r = q.det_begin + rr;
     const long long v = vox ? (long long)vox[j] : (long long)j;
+    const float4* kr = q.ring + 2LL * r * q.n_keep;
+    const int cnt = keep_cnt[r];
+    const float* tab = q.ftab + (long long)r * (kSgTab + 1);
+    float xI, yI, zI;
+    sg_position(q, v, xI, yI, zI);
+    const float ring = sg_ring(q, kr, q.n_near, cnt, tab, pe, v, xI, yI, zI) +
+                       sg_ring(q, kr, q.n_near + 1, cnt, tab, pe, v, xI, yI, zI);
     float f[kSgFeatures];
-    sg_features(q, rows[r], keep + (long long)r * q.n_keep, keep_cnt[r], r, v, pe, near[rr * near_stride + v], f);
+    sg_finish(q, rows[r], r, v, pe, near[rr * near_stride + v], ring, xI, yI, zI, f);
 #pragma unroll
     for (int k = 0; k < kSgFeatures; ++k) x[i * kSgFeatures + k] = f[k];
 }
@@ -80,18 +128,19 @@ __device__ inline long long sg_voxel(const SgParams& q, long long ch, int i, boo
 
 // One wave = half a 4 x 4 x 4 voxel brick (32 voxels) of one detector row.
 // Features go through LDS into the MFMA B-operand layout; every member runs
-// with register-resident activations and weight slabs shared by the
-// workgroup's four waves through LDS (every wave runs every member).
-// Disagreeing members flag the brick (tile) for the exact kernel, which prunes
-// scatter sites per brick.
-__global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
+// with register-resident activations and weight tiles streamed from L2 (each
+// wave on its own: sharing weight slabs through LDS cost two workgroup barriers
+// per slab, more than the L2 reads it saved).  Disagreeing members flag the
+// brick (tile) for the exact kernel, which prunes scatter sites per brick.
+// Three waves per SIMD (168 VGPRs, a few spilled) hide the ring prior's PE
+// gathers behind the other waves' MFMAs better than two without spills.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
                                                               const ScPair* __restrict__ keep,
                                                               const int* __restrict__ keep_cnt,
                                                               const uint8_t* __restrict__ packed,
                                                               const float* __restrict__ pe, float* __restrict__ dst,
                                                               uint8_t* __restrict__ flags) {
     __shared__ float s_f[4][32][kSgFeatures + 1];
-    __shared__ uint4 s_w[16 * 64];  // one weight slab, shared by the four waves
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j16 = lane & 15;
     const long long chunks = 2 * q.n_tiles;
     const long long gw = (long long)blockIdx.x * 4 + w;
@@ -101,18 +150,30 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
     const int r = q.det_begin + (valid_wave ? rr : 0);
     const SgRow R = rows[r];
     const bool use = valid_wave && R.use != 0.0f;
-    if (lane < 32) {
+    {
+        // lanes e and e + 32 share element e: even and odd ring sites
+        const int e = lane & 31, half = lane >> 5;
         bool ok;
-        const long long v = sg_voxel(q, ch, lane, ok);
-        float f[kSgFeatures];
-        if (use && ok) {
-            sg_features(q, R, keep + (long long)r * q.n_keep, keep_cnt[r], r, v, pe, dst[rr * q.row_stride + v], f);
-        } else {
-#pragma unroll
-            for (int k = 0; k < kSgFeatures; ++k) f[k] = 0.0f;
+        const long long v = sg_voxel(q, ch, e, ok);
+        const bool live = use && ok;
+        float xI = 0.0f, yI = 0.0f, zI = 0.0f, ring = 0.0f;
+        if (live) {
+            sg_position(q, v, xI, yI, zI);
+            ring = sg_ring(q, q.ring + 2LL * r * q.n_keep, q.n_near + half, keep_cnt[r],
+                           q.ftab + (long long)r * (kSgTab + 1), pe, v, xI, yI, zI);
         }
+        const float other = __shfl_xor(ring, 32, 64);
+        if (half == 0) {
+            float f[kSgFeatures];
+            if (live) {
+                sg_finish(q, R, r, v, pe, dst[rr * q.row_stride + v], ring + other, xI, yI, zI, f);
+            } else {
 #pragma unroll
-        for (int k = 0; k < kSgFeatures; ++k) s_f[w][lane][k] = f[k];
+                for (int k = 0; k < kSgFeatures; ++k) f[k] = 0.0f;
+            }
+#pragma unroll
+            for (int k = 0; k < kSgFeatures; ++k) s_f[w][e][k] = f[k];
+        }
     }
     __syncthreads();
     bf16x8 xf[2];
@@ -126,7 +187,7 @@ __global__ void __launch_bounds__(256) surrogate_apply_kernel(SgParams q, const 
     float far_sum[2] = {0.f, 0.f}, far_min[2] = {3.0e38f, 3.0e38f}, far_max[2] = {0.f, 0.f};
     for (int m = 0; m < q.n_members; ++m) {
         float y[2];
-        mlp_block_forward(packed + (long long)m * kMlpPacked, xf, y, s_w);
+        mlp_wave_forward(packed + (long long)m * kMlpPacked, xf, y);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
             const float fr = fmaxf(__expf(y[t] * q.sig_y + q.mu_y) - q.phi, 0.0f);
