@@ -448,8 +448,9 @@ __device__ inline double site_weight(const ScParams& p, const ScRow& R, const Sc
 
 // Contribution of one site to element (r, voxel) with source weight w
 // (scatter.cu:1240-1427 / 1736-1923).
+// kn: the (cos n h, sin n h) table (a workgroup's LDS copy in the scatter kernel).
 __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPair& S, const VoxelPt& q,
-                                   const KnCoef& kc, double w) {
+                                   const KnCoef& kc, double w, const double2* __restrict__ kn) {
     const double e0 = p.e0, ea = p.e0 / 511.0;
     const double two_pi = 2.0 * M_PI;
     // cone angle at the site (scatter.cu:996-1015, 1246-1248)
@@ -503,7 +504,7 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
         dth = atan2(smax * cmax - cmin * sa, cmin * cmax + smax * sa);
     }
     const int nstep = min(kKnSteps, max(0, (int)(dth / p.h)));
-    const double2 cs = p.kn_tab[nstep];
+    const double2 cs = kn[nstep];
     const double ratio = kn_sum_cs(kc, ua, sa, cs.x, cs.y, p.h) / p.kn_total;
     // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
     const double x = e0 / es, x3 = x * x * x;
@@ -514,10 +515,21 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
            (pe_abs / tot_abs);
 }
 
-__global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* __restrict__ rows,
+// Four waves per SIMD (128 VGPRs, ~116 B/lane spilled): the per-site PE gather and
+// the FP64 chains of the site term need the extra waves to hide behind; measured
+// against 2 (176 VGPRs, no spill): -29 % on the C3 crystal scatter, -37 % on rows
+// of the 16384-crystal detector; 3 waves were in between, 5-8 waves spill more
+// and lose again.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) scatter_kernel(ScParams p, const ScRow* __restrict__ rows,
                                                       const ScPair* __restrict__ pairs,
                                                       const int* __restrict__ counts, const float* __restrict__ pe,
                                                       float* __restrict__ dst) {
+    // the KN end-angle table in LDS: its per-site lookup depends on the voxel's theta
+    // range, so from global memory it is a dependent gather in every site iteration
+    // (staged before any wave can leave, so every wave reaches the barrier)
+    __shared__ double2 s_kn[kKnSteps + 1];
+    for (int i = threadIdx.x; i <= kKnSteps; i += blockDim.x) s_kn[i] = p.kn_tab[i];
+    __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     int rr, tile;
     if (p.items) {
@@ -603,7 +615,7 @@ __global__ void __launch_bounds__(256) scatter_kernel(ScParams p, const ScRow* _
                 }
                 if (!active) continue;
                 const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
-                acc += site_term(p, R, S, q, kc, w);
+                acc += site_term(p, R, S, q, kc, w, s_kn);
             }
         }
     }
@@ -657,7 +669,7 @@ __global__ void __launch_bounds__(256) surrogate_rank_kernel(ScParams p, Surroga
         const int s = k.cand[(long long)r * k.n_cand + t];
         if (s >= 0 && s != r && make_pair(p, R, r, s, cry, nullptr, bits, B, P)) {
             const VoxelPt q = voxel_point(p, R, p.sx, p.sy, p.sz);
-            const double g = site_term(p, R, P, q, kn_coef(p.e0), (double)(float)(R.mu_c / R.mu_pe));
+            const double g = site_term(p, R, P, q, kn_coef(p.e0), (double)(float)(R.mu_c / R.mu_pe), p.kn_tab);
             key = g > 0.0 ? (float)g : 0.0f;
             P.pad = key;
             // weight without its angular factor at the image centre (voxel-dependent ring prior)
