@@ -16,14 +16,20 @@
 namespace sysmat {
 
 // ---------------------------------------------------------------------------
-// GEMM: C(m,n) = sum_k A(m,k) B(k,n), bf16 operands staged through LDS
-// (k-contiguous rows, padded to 40 elements), 64x64 block tile, 4 waves of
-// 32x32 (2x2 MFMA 16x16x32), FP32 accumulation, split-K over blockIdx.z.
+// GEMM: C(m,n) = sum_k A(m,k) B(k,n), bf16 operands staged through LDS in
+// k-stages of up to 128 (k-contiguous rows, padded by 8 elements), 64x64 block
+// tile, 4 waves of 32x32 (2x2 MFMA 16x16x32), FP32 accumulation, split-K over
+// blockIdx.z.  The training GEMMs are small (K = 32 or 256, or 128 per split),
+// so a wide stage means one or two global round trips per tile instead of
+// eight.  K ranges are multiples of 32.
 // ---------------------------------------------------------------------------
+constexpr int kGemmKS = 128;
+constexpr int kGemmLd = kGemmKS + 8;
+
 template <bool AK, bool BK, int EPI>
 __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
-    __shared__ __attribute__((aligned(16))) bf16 As[64][40];
-    __shared__ __attribute__((aligned(16))) bf16 Bs[64][40];
+    __shared__ __attribute__((aligned(16))) bf16 As[64][kGemmLd];
+    __shared__ __attribute__((aligned(16))) bf16 Bs[64][kGemmLd];
     const int tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const int n0 = blockIdx.x * 64, m0 = blockIdx.y * 64;
     const int kbeg = blockIdx.z * g.ksplit, kend = min(g.K, kbeg + g.ksplit);
@@ -34,43 +40,59 @@ __global__ void __launch_bounds__(256) gemm_kernel(GemmArgs g) {
     for (int i = 0; i < 2; ++i)
 #pragma unroll
         for (int j = 0; j < 2; ++j) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
-    for (int k0 = kbeg; k0 < kend; k0 += 32) {
-        if (AK) {
-            const int row = tid >> 2, k8 = (tid & 3) * 8, m = m0 + row;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (m < g.M) v = *reinterpret_cast<const uint4*>(g.A + (long long)m * g.lda + k0 + k8);
-            *reinterpret_cast<uint4*>(&As[row][k8]) = v;
-        } else {
-            const int k = tid >> 3, m8 = (tid & 7) * 8, m = m0 + m8;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (m < g.M) v = *reinterpret_cast<const uint4*>(g.A + (long long)(k0 + k) * g.lda + m);
-            const bf16x8 e = __builtin_bit_cast(bf16x8, v);
+    for (int k0 = kbeg; k0 < kend; k0 += kGemmKS) {
+        const int nk = min(kGemmKS, kend - k0);
+        // 64 x nk of each operand: (64 * 128 / 8) / 256 = 4 uint4 per thread
 #pragma unroll
-            for (int i = 0; i < 8; ++i) As[m8 + i][k] = e[i];
-        }
-        if (BK) {
-            const int col = tid >> 2, k8 = (tid & 3) * 8, n = n0 + col;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (n < g.N) v = *reinterpret_cast<const uint4*>(g.B + (long long)n * g.ldb + k0 + k8);
-            *reinterpret_cast<uint4*>(&Bs[col][k8]) = v;
-        } else {
-            const int k = tid >> 3, n8 = (tid & 7) * 8, n = n0 + n8;
-            uint4 v = {0u, 0u, 0u, 0u};
-            if (n < g.N) v = *reinterpret_cast<const uint4*>(g.B + (long long)(k0 + k) * g.ldb + n);
-            const bf16x8 e = __builtin_bit_cast(bf16x8, v);
+        for (int it = 0; it < 4; ++it) {
+            const int idx = tid + 256 * it;
+            if (AK) {
+                const int row = idx >> 4, k8 = (idx & 15) * 8, m = m0 + row;
+                if (k8 < nk) {
+                    uint4 v = {0u, 0u, 0u, 0u};
+                    if (m < g.M) v = *reinterpret_cast<const uint4*>(g.A + (long long)m * g.lda + k0 + k8);
+                    *reinterpret_cast<uint4*>(&As[row][k8]) = v;
+                }
+            } else {
+                const int k = idx >> 3, m8 = (idx & 7) * 8, m = m0 + m8;
+                if (k < nk) {
+                    uint4 v = {0u, 0u, 0u, 0u};
+                    if (m < g.M) v = *reinterpret_cast<const uint4*>(g.A + (long long)(k0 + k) * g.lda + m);
+                    const bf16x8 e = __builtin_bit_cast(bf16x8, v);
 #pragma unroll
-            for (int i = 0; i < 8; ++i) Bs[n8 + i][k] = e[i];
+                    for (int i = 0; i < 8; ++i) As[m8 + i][k] = e[i];
+                }
+            }
+            if (BK) {
+                const int col = idx >> 4, k8 = (idx & 15) * 8, n = n0 + col;
+                if (k8 < nk) {
+                    uint4 v = {0u, 0u, 0u, 0u};
+                    if (n < g.N) v = *reinterpret_cast<const uint4*>(g.B + (long long)n * g.ldb + k0 + k8);
+                    *reinterpret_cast<uint4*>(&Bs[col][k8]) = v;
+                }
+            } else {
+                const int k = idx >> 3, n8 = (idx & 7) * 8, n = n0 + n8;
+                if (k < nk) {
+                    uint4 v = {0u, 0u, 0u, 0u};
+                    if (n < g.N) v = *reinterpret_cast<const uint4*>(g.B + (long long)(k0 + k) * g.ldb + n);
+                    const bf16x8 e = __builtin_bit_cast(bf16x8, v);
+#pragma unroll
+                    for (int i = 0; i < 8; ++i) Bs[n8 + i][k] = e[i];
+                }
+            }
         }
         __syncthreads();
-        bf16x8 a[2], b[2];
+        for (int kb = 0; kb < nk; kb += 32) {
+            bf16x8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + l16][8 * g4]);
+            for (int i = 0; i < 2; ++i) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm + 16 * i + l16][kb + 8 * g4]);
 #pragma unroll
-        for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + l16][8 * g4]);
+            for (int j = 0; j < 2; ++j) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn + 16 * j + l16][kb + 8 * g4]);
 #pragma unroll
-        for (int i = 0; i < 2; ++i)
+            for (int i = 0; i < 2; ++i)
 #pragma unroll
-            for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+                for (int j = 0; j < 2; ++j) acc[i][j] = mfma16(a[i], b[j], acc[i][j]);
+        }
         __syncthreads();
     }
     // acc[i][j][r]: row m0 + wm + 16i + 4*g4 + r, column n0 + wn + 16j + l16
@@ -132,10 +154,13 @@ __global__ void pack_input_kernel(const float* __restrict__ x, long long n, long
 
 // y[b] = w5 . silu(Z4[b]) + b5; with targets: dy = 2 w (y - t), loss partial
 // (one wave per row; rows >= n give y = 0, dy = 0).
+// wpart (nullable): n_wpart partial sums of the raw weights wt, added in order;
+// the weights are then wt / total (normalised batch weights).
 __global__ void __launch_bounds__(256) head_fwd_kernel(const float* __restrict__ P, const float* __restrict__ Z4,
                                                        long long n, long long B, const float* __restrict__ t,
                                                        const float* __restrict__ wt, float* __restrict__ y,
-                                                       float* __restrict__ dy, float* __restrict__ loss_part) {
+                                                       float* __restrict__ dy, float* __restrict__ loss_part,
+                                                       const float* __restrict__ wpart, int n_wpart) {
     __shared__ float s_loss[4];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     const long long b = (long long)blockIdx.x * 4 + w;
@@ -157,7 +182,12 @@ __global__ void __launch_bounds__(256) head_fwd_kernel(const float* __restrict__
                 float d = 0.0f;
                 if (b < n) {
                     const float r = yy - t[b];
-                    const float ww = wt ? wt[b] : 1.0f / (float)n;
+                    float ww = wt ? wt[b] : 1.0f / (float)n;
+                    if (wpart) {
+                        float tot = 0.0f;
+                        for (int q = 0; q < n_wpart; ++q) tot += wpart[q];
+                        ww *= 1.0f / tot;
+                    }
                     d = 2.0f * ww * r;
                     l = ww * r * r;
                 }
@@ -229,16 +259,20 @@ __global__ void reduce_w_kernel(const float* __restrict__ part, int S, int M, lo
     out[m * ld_out + c] = s;
 }
 
+// pb (nullable): the bf16 copy of the updated parameters for the next step's GEMMs
 __global__ void adam_kernel(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m,
-                            float* __restrict__ v, long long n, float lr_t, float b1, float b2, float eps) {
+                            float* __restrict__ v, long long n, float lr_t, float b1, float b2, float eps,
+                            bf16* __restrict__ pb) {
     const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const float gi = gr[i];
+    const float gi = i <= kOffB5 ? gr[i] : 0.0f;  // the padding after b5 has no gradient
     const float mi = b1 * m[i] + (1.0f - b1) * gi;
     const float vi = b2 * v[i] + (1.0f - b2) * gi * gi;
     m[i] = mi;
     v[i] = vi;
-    p[i] -= lr_t * mi / (sqrtf(vi) + eps);
+    const float pn = p[i] - lr_t * mi / (sqrtf(vi) + eps);
+    p[i] = pn;
+    if (pb) pb[i] = (bf16)pn;
 }
 
 __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
@@ -246,6 +280,49 @@ __device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
     z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
     z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
     return z ^ (z >> 31);
+}
+
+// Batch of training step `step` drawn on the device: row b takes pair
+// mix64(mix64(seed ^ K step) + b) mod n (with replacement).  One thread per row
+// gathers the features into the bf16 GEMM layout (12 of 32 columns, rest zero),
+// the target and the raw weight (w NULL: 1); each 256-row block leaves the sum of
+// its weights (wave xor-butterflies, then the four waves in order) in wpart,
+// which the head kernel adds in block order to normalise the weights.
+constexpr int kSampleRows = 256;
+__global__ void __launch_bounds__(kSampleRows) sample_batch_kernel(const float* __restrict__ x,
+                                                                   const float* __restrict__ t,
+                                                                   const float* __restrict__ w, long long n, int B,
+                                                                   unsigned long long seed, long long step,
+                                                                   bf16* __restrict__ X, float* __restrict__ tb,
+                                                                   float* __restrict__ wb, float* __restrict__ wpart) {
+    __shared__ float s_w[kSampleRows / 64];
+    const int b = blockIdx.x * kSampleRows + threadIdx.x;
+    float wi = 0.0f;
+    if (b < B) {
+        const unsigned long long h = mix64(mix64(seed ^ (0x632BE59BD9B4E019ull * (unsigned long long)step)) +
+                                           (unsigned long long)b);
+        const long long i = (long long)(h % (unsigned long long)n);
+        const float* xi = x + i * kMlpIn;
+        bf16x8 e[4];
+#pragma unroll
+        for (int k = 0; k < kMlpK1; ++k) e[k >> 3][k & 7] = (bf16)(k < kMlpIn ? xi[k] : 0.0f);
+        uint4* dst = reinterpret_cast<uint4*>(X + (long long)b * kMlpK1);
+#pragma unroll
+        for (int q = 0; q < 4; ++q) dst[q] = __builtin_bit_cast(uint4, e[q]);
+        tb[b] = t[i];
+        wi = w ? w[i] : 1.0f;
+        wb[b] = wi;
+    }
+    float ws = wi;
+    for (int o = 32; o > 0; o >>= 1) ws += __shfl_xor(ws, o, 64);
+    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = ws;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float tot = 0.0f;
+#pragma unroll
+        for (int q = 0; q < kSampleRows / 64; ++q) tot += s_w[q];
+        wpart[blockIdx.x] = tot;
+    }
 }
 
 // uniform +-sqrt(6/fan_in) weights, zero biases, w5 uniform +-1/16, zero W1 padding
@@ -345,6 +422,12 @@ hipError_t launch_mlp_infer(const uint8_t* packed, int n_members, const float* x
 namespace {
 
 constexpr int kHeadRows = 64;  // rows per block of the head / column-sum reductions
+constexpr float kAdamB1 = 0.9f, kAdamB2 = 0.999f, kAdamEps = 1e-8f;
+
+// Adam step size with the bias corrections of step `step` (counting from 1)
+float adam_lr(float lr, int step) {
+    return lr * std::sqrt(1.0f - std::pow(kAdamB2, (float)step)) / (1.0f - std::pow(kAdamB1, (float)step));
+}
 
 struct Ws {
     bf16* Pb;
@@ -357,6 +440,9 @@ struct Ws {
     float* part;
     float* loss_part;
     float* grad;
+    float* tb;  // sampled batch targets / raw weights / per-block weight sums (multi-step training)
+    float* wb;
+    float* wpart;
     long long B;
     int splits;
 };
@@ -391,6 +477,9 @@ long long ws_bytes(long long B, Ws* w, uint8_t* base) {
     uint8_t* part = take(part_floats * 4);
     uint8_t* lp = take(((B + 3) / 4) * 4);
     uint8_t* gr = take(kMlpParams * 4);
+    uint8_t* tb = take(B * 4);
+    uint8_t* wb = take(B * 4);
+    uint8_t* wp = take(((B + kSampleRows - 1) / kSampleRows) * 4);
     if (w) {
         w->Pb = (bf16*)pb;
         w->X = (bf16*)x;
@@ -402,6 +491,9 @@ long long ws_bytes(long long B, Ws* w, uint8_t* base) {
         w->part = (float*)part;
         w->loss_part = (float*)lp;
         w->grad = (float*)gr;
+        w->tb = (float*)tb;
+        w->wb = (float*)wb;
+        w->wpart = (float*)wp;
         w->B = B;
         w->splits = S;
     }
@@ -415,12 +507,15 @@ int check_batch(int n, long long B) {
     return SM_OK;
 }
 
-// forward with activations kept for backward; x: [n][12] FP32 (rows n..B-1 zero)
+// forward with activations kept for backward; x: [n][12] FP32 (rows n..B-1 zero);
+// x NULL: w.X and w.Pb already hold the packed batch and the bf16 parameters
 int forward_train(const float* P, const float* x, int n, Ws& w, hipStream_t s) {
     const long long B = w.B;
-    hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(kMlpParams)), dim3(256), 0, s, P, w.Pb, kMlpParams);
-    hipLaunchKernelGGL(pack_input_kernel, dim3(nblk(B * kMlpK1)), dim3(256), 0, s, x, (long long)n, B, w.X);
-    HIP_TRY(hipGetLastError());
+    if (x) {
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(kMlpParams)), dim3(256), 0, s, P, w.Pb, kMlpParams);
+        hipLaunchKernelGGL(pack_input_kernel, dim3(nblk(B * kMlpK1)), dim3(256), 0, s, x, (long long)n, B, w.X);
+        HIP_TRY(hipGetLastError());
+    }
     const long long woff[4] = {kOffW1, kOffW2, kOffW3, kOffW4};
     const long long boff[4] = {kOffB1, kOffB2, kOffB3, kOffB4};
     for (int l = 0; l < 4; ++l) {
@@ -537,7 +632,7 @@ int sm_mlp_forward(const float* params_dev, const float* x_dev, int n, int batch
     if ((rc = forward_train(params_dev, x_dev, n, w, s)) < 0) return rc;
     hipLaunchKernelGGL(head_fwd_kernel, dim3(nblk(batch, 4)), dim3(256), 0, s, params_dev,
                        w.Z + 3LL * batch * kMlpH, (long long)n, (long long)batch, nullptr, nullptr, y_dev, nullptr,
-                       nullptr);
+                       nullptr, nullptr, 0);
     HIP_TRY(hipGetLastError());
     return SM_OK;
 }
@@ -573,19 +668,101 @@ int sm_mlp_train_step(float* params_dev, float* adam_m, float* adam_v, const flo
     if ((rc = forward_train(params_dev, x_dev, n, w, s)) < 0) return rc;
     const unsigned hb = nblk(batch, 4);
     hipLaunchKernelGGL(head_fwd_kernel, dim3(hb), dim3(256), 0, s, params_dev, w.Z + 3LL * batch * kMlpH,
-                       (long long)n, (long long)batch, t_dev, w_dev, w.y, w.dy, w.loss_part);
+                       (long long)n, (long long)batch, t_dev, w_dev, w.y, w.dy, w.loss_part, nullptr, 0);
     HIP_TRY(hipGetLastError());
     if (loss_dev) {
         hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(1), 0, s, w.loss_part, (int)hb, 1LL, 1LL, loss_dev);
         HIP_TRY(hipGetLastError());
     }
-    HIP_TRY(hipMemsetAsync(w.grad, 0, sizeof(float) * kMlpParams, s));
+    // every gradient entry up to b5 is written (not accumulated) by the reductions of
+    // backward(); Adam reads the padding after b5 as zero
     if ((rc = backward(params_dev, n, w, s)) < 0) return rc;
-    const float b1 = 0.9f, b2 = 0.999f;
-    const float lr_t = lr * std::sqrt(1.0f - std::pow(b2, (float)step)) / (1.0f - std::pow(b1, (float)step));
     hipLaunchKernelGGL(adam_kernel, dim3(nblk(kMlpParams)), dim3(256), 0, s, params_dev, w.grad, adam_m, adam_v,
-                       kMlpParams, lr_t, b1, b2, 1e-8f);
+                       kMlpParams, adam_lr(lr, step), kAdamB1, kAdamB2, kAdamEps, (bf16*)nullptr);
     HIP_TRY(hipGetLastError());
+    return SM_OK;
+}
+
+int sm_mlp_train_run(int n_members, float* const* params_dev, float* const* adam_m, float* const* adam_v,
+                     const float* x_dev, const float* t_dev, const float* w_dev, long long n, int batch, int steps,
+                     float lr, float lr_floor, const unsigned long long* seeds, int step0, void* const* ws,
+                     long long ws_size, float* const* loss_dev, void* stream) {
+    clear_error();
+    if (n_members < 1 || n_members > 8) return set_error(SM_E_ARG, "mlp: 1 to 8 members per run");
+    if (n <= 0 || batch <= 0 || batch % 64 != 0) return set_error(SM_E_ARG, "mlp: need n > 0 and a batch that is a positive multiple of 64");
+    if (steps < 0 || step0 < 0) return set_error(SM_E_ARG, "mlp: negative step count");
+    if (!params_dev || !adam_m || !adam_v || !x_dev || !t_dev || !seeds || !ws) return set_error(SM_E_ARG, "null argument");
+    Ws w[8];
+    for (int m = 0; m < n_members; ++m) {
+        if (!params_dev[m] || !adam_m[m] || !adam_v[m] || !ws[m]) return set_error(SM_E_ARG, "null argument");
+        if (ws_bytes(batch, &w[m], (uint8_t*)ws[m]) > ws_size) return set_error(SM_E_ARG, "mlp: workspace too small");
+    }
+    // member m runs on its own stream (member 0 on the caller's), forked from and joined
+    // back to the caller's stream: the members' short, latency-bound kernels overlap
+    hipStream_t s0 = (hipStream_t)stream;
+    hipStream_t st[8] = {s0};
+    hipEvent_t fork = nullptr, done[8] = {};
+    // (destroying a stream or event with work pending is deferred by the runtime until
+    // that work completes, so cleanup never waits)
+    auto cleanup = [&]() {
+        for (int m = 1; m < n_members; ++m) {
+            if (done[m]) (void)hipEventDestroy(done[m]);
+            if (st[m]) (void)hipStreamDestroy(st[m]);
+        }
+        if (fork) (void)hipEventDestroy(fork);
+    };
+    auto fail = [&](int rc) {
+        cleanup();
+        return rc;
+    };
+    if (n_members > 1) {
+        if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: event"));
+        if (hipEventRecord(fork, s0) != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: event record"));
+        for (int m = 1; m < n_members; ++m) {
+            if (hipStreamCreateWithFlags(&st[m], hipStreamNonBlocking) != hipSuccess ||
+                hipEventCreateWithFlags(&done[m], hipEventDisableTiming) != hipSuccess ||
+                hipStreamWaitEvent(st[m], fork, 0) != hipSuccess)
+                return fail(set_error(SM_E_DEVICE, "mlp: member stream"));
+        }
+    }
+    const unsigned hb = nblk(batch, 4);
+    const int nwp = (batch + kSampleRows - 1) / kSampleRows;
+    int rc;
+    for (int m = 0; m < n_members; ++m) {
+        hipLaunchKernelGGL(to_bf16_kernel, dim3(nblk(kMlpParams)), dim3(256), 0, st[m], params_dev[m], w[m].Pb,
+                           kMlpParams);
+        if (hipGetLastError() != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: launch"));
+    }
+    for (int it = 0; it < steps; ++it) {
+        const int step = step0 + it + 1;
+        // cosine schedule from lr over the run, held at lr_floor from below
+        const float lr_it = std::max(lr * 0.5f * (1.0f + std::cos(3.14159265358979f * (float)it / (float)steps)),
+                                     lr_floor);
+        for (int m = 0; m < n_members; ++m) {
+            Ws& W = w[m];
+            hipStream_t s = st[m];
+            hipLaunchKernelGGL(sample_batch_kernel, dim3(nwp), dim3(kSampleRows), 0, s, x_dev, t_dev, w_dev, n,
+                               batch, seeds[m], (long long)step, W.X, W.tb, W.wb, W.wpart);
+            if (hipGetLastError() != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: launch"));
+            if ((rc = forward_train(params_dev[m], nullptr, batch, W, s)) < 0) return fail(rc);
+            hipLaunchKernelGGL(head_fwd_kernel, dim3(hb), dim3(256), 0, s, params_dev[m],
+                               W.Z + 3LL * batch * kMlpH, (long long)batch, (long long)batch, W.tb, W.wb, W.y, W.dy,
+                               W.loss_part, W.wpart, nwp);
+            if (loss_dev && loss_dev[m] && it == steps - 1)
+                hipLaunchKernelGGL(reduce_parts_kernel, dim3(1), dim3(1), 0, s, W.loss_part, (int)hb, 1LL, 1LL,
+                                   loss_dev[m]);
+            if (hipGetLastError() != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: launch"));
+            if ((rc = backward(params_dev[m], batch, W, s)) < 0) return fail(rc);
+            hipLaunchKernelGGL(adam_kernel, dim3(nblk(kMlpParams)), dim3(256), 0, s, params_dev[m], W.grad, adam_m[m],
+                               adam_v[m], kMlpParams, adam_lr(lr_it, step), kAdamB1, kAdamB2, kAdamEps, W.Pb);
+            if (hipGetLastError() != hipSuccess) return fail(set_error(SM_E_DEVICE, "mlp: launch"));
+        }
+    }
+    for (int m = 1; m < n_members; ++m) {
+        if (hipEventRecord(done[m], st[m]) != hipSuccess || hipStreamWaitEvent(s0, done[m], 0) != hipSuccess)
+            return fail(set_error(SM_E_DEVICE, "mlp: member join"));
+    }
+    cleanup();
     return SM_OK;
 }
 

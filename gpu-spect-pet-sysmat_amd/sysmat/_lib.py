@@ -123,6 +123,9 @@ SIGNATURES = {
     "sm_mlp_backward": (ctypes.c_int, [_VP, _VP, ctypes.c_int, ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
     "sm_mlp_train_step": (ctypes.c_int, [_VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_int, ctypes.c_int, ctypes.c_float,
                                          ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
+    "sm_mlp_train_run": (ctypes.c_int, [ctypes.c_int, _VP, _VP, _VP, _VP, _VP, _VP, ctypes.c_longlong, ctypes.c_int,
+                                        ctypes.c_int, ctypes.c_float, ctypes.c_float, _VP, ctypes.c_int, _VP,
+                                        ctypes.c_longlong, _VP, _VP]),
     "sm_mlp_pack": (ctypes.c_int, [_VP, _VP, _VP]),
     "sm_mlp_infer": (ctypes.c_int, [_VP, ctypes.c_int, _VP, ctypes.c_longlong, _VP, _VP]),
 }

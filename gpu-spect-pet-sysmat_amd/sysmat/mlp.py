@@ -99,12 +99,54 @@ class Mlp:
                                        ptr(self._loss) if want_loss else None, _stream_handle()))
         return self._loss if want_loss else None
 
+    def train_run(self, x, t, w=None, steps: int = 1000, lr: float = 1e-3, lr_floor: float = 0.0, seed: int = 0,
+                  batch: int = 4096, want_loss: bool = False):
+        """``steps`` Adam steps on batches of ``batch`` rows drawn on the device from the n
+        training pairs (x [n, 12], t [n], optional weights w [n]) with a cosine learning-rate
+        schedule from ``lr`` held at ``lr_floor``; returns the last batch's loss (device
+        scalar) when ``want_loss``."""
+        losses = train_members([self], x, t, w, steps=steps, lr=lr, lr_floor=lr_floor, seeds=[seed], batch=batch,
+                               want_loss=want_loss)
+        return losses[0] if want_loss else None
+
     def pack(self, out=None):
         import torch
         if out is None:
             out = torch.empty(packed_bytes(), dtype=torch.uint8, device=self.device)
         check(self.L.sm_mlp_pack(ptr(self.params), ptr(out), _stream_handle()))
         return out
+
+
+def train_members(nets: Sequence[Mlp], x, t, w=None, steps: int = 1000, lr: float = 1e-3, lr_floor: float = 0.0,
+                  seeds: Sequence[int] = (0,), batch: int = 4096, want_loss: bool = False):
+    """Train the networks of an ensemble together (each on its own stream, its own batch
+    seed, all at the same step count): the per-step kernels are short and latency-bound,
+    so the members' chains overlap on the GPU.  Every member's result is the one it would
+    reach alone.  Returns the members' last-batch losses (device scalars) when ``want_loss``."""
+    L = load()
+    nets = list(nets)
+    k = len(nets)
+    if len(seeds) != k:
+        raise ValueError("one seed per member")
+    step0 = nets[0].step
+    if any(nt.step != step0 for nt in nets):
+        raise ValueError("members must be at the same step count")
+    x, t = x.contiguous(), t.contiguous()
+    w = None if w is None else w.contiguous()
+    wss = [nt._workspace(batch) for nt in nets]
+    arr = ctypes.c_void_p * k
+    params = arr(*[nt.params.data_ptr() for nt in nets])
+    ms = arr(*[nt.m.data_ptr() for nt in nets])
+    vs = arr(*[nt.v.data_ptr() for nt in nets])
+    wsp = arr(*[ws.data_ptr() for ws in wss])
+    losses = arr(*[nt._loss.data_ptr() for nt in nets]) if want_loss else None
+    sd = (ctypes.c_ulonglong * k)(*[int(s) for s in seeds])
+    check(L.sm_mlp_train_run(k, params, ms, vs, ptr(x), ptr(t), None if w is None else ptr(w), int(x.shape[0]),
+                             int(batch), int(steps), float(lr), float(lr_floor), sd, int(step0), wsp,
+                             min(ws.numel() for ws in wss), losses, _stream_handle()))
+    for nt in nets:
+        nt.step += steps
+    return [nt._loss for nt in nets] if want_loss else None
 
 
 def pack_members(members: Sequence[Mlp]):

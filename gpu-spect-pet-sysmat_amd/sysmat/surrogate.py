@@ -198,24 +198,17 @@ class ScatterSurrogate:
         B = min(cfg.batch, (n // 64) * 64) if n >= 64 else 64
         self.members = []
         losses = []
-        for m in range(cfg.members):
-            net = mlp.Mlp(seed=cfg.seed * 1000 + 17 + m, device=str(dev))
-            gen = torch.Generator(device=dev).manual_seed(cfg.seed * 7919 + m)
-            for it in range(cfg.steps):
-                idx = torch.randint(0, n, (B,), device=dev, generator=gen)
-                wb = w.index_select(0, idx)
-                wb = (wb / wb.sum()).contiguous()
-                lr = cfg.lr * 0.5 * (1.0 + np.cos(np.pi * it / cfg.steps))  # cosine schedule
-                want = (it == cfg.steps - 1) or (log is not None and it % 500 == 0)
-                loss = net.train_step(x.index_select(0, idx).contiguous(), t.index_select(0, idx).contiguous(), wb,
-                                      lr=max(lr, cfg.lr * 0.01), batch=B, want_loss=want)
-                if want:
-                    lv = float(loss.item())
-                    if it == cfg.steps - 1:
-                        losses.append(lv)
-                    if log:
-                        log(f"member {m} step {it}: weighted loss {lv:.4e}")
-            self.members.append(net)
+        nets = [mlp.Mlp(seed=cfg.seed * 1000 + 17 + m, device=str(dev)) for m in range(cfg.members)]
+        # cosine schedule from cfg.lr over cfg.steps, held at 1 % of it; batches drawn on the
+        # device; the members train concurrently
+        ls = mlp.train_members(nets, x, t, w, steps=cfg.steps, lr=cfg.lr, lr_floor=cfg.lr * 0.01,
+                               seeds=[cfg.seed * 7919 + m for m in range(cfg.members)], batch=B, want_loss=True)
+        for m, loss in enumerate(ls):
+            lv = float(loss.item())
+            losses.append(lv)
+            if log:
+                log(f"member {m}: {cfg.steps} steps, last batch weighted loss {lv:.4e}")
+        self.members = nets
         self.packed = mlp.pack_members(self.members)
         torch.cuda.synchronize()
         t3 = time.perf_counter()

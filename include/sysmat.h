@@ -247,6 +247,19 @@ int sm_mlp_backward(const float* params_dev, const float* dy_dev, int n, int bat
 int sm_mlp_train_step(float* params_dev, float* adam_m, float* adam_v, const float* x_dev, const float* t_dev,
                       const float* w_dev, int n, int batch, float lr, int step, void* ws, long long ws_size,
                       float* loss_dev, void* stream);
+/* `steps` Adam steps for each of n_members (1..8) networks (member m: its
+ * params/adam_m/adam_v/ws, batch seed seeds[m]) on batches drawn on the device:
+ * step s = step0 + 1 .. step0 + steps takes `batch` rows of the n training pairs
+ * (x_dev [n][12], t_dev [n], w_dev [n] or NULL for equal weights) at indices
+ * hashed from (seeds[m], s), the weights normalised to sum 1 per batch;
+ * learning rate max(lr (1 + cos(pi i / steps)) / 2, lr_floor) at step i of the
+ * run.  No host round trip per step; the members run concurrently on their own
+ * streams, joined back to `stream` before return.  loss_dev (nullable, or
+ * per-member entries NULL) gets each member's last-batch loss. */
+int sm_mlp_train_run(int n_members, float* const* params_dev, float* const* adam_m, float* const* adam_v,
+                     const float* x_dev, const float* t_dev, const float* w_dev, long long n, int batch, int steps,
+                     float lr, float lr_floor, const unsigned long long* seeds, int step0, void* const* ws,
+                     long long ws_size, float* const* loss_dev, void* stream);
 /* Repack parameters for inference (sm_mlp_packed_bytes() bytes per member). */
 int sm_mlp_pack(const float* params_dev, void* packed_dev, void* stream);
 /* Register-resident inference of n rows for n_members consecutive packed

@@ -117,3 +117,118 @@ def test_training_reduces_loss(lib):
     final = float(((net.forward(xd) - td) ** 2).mean().item())
     print(f"final mse {final:.3e} (start {first:.3e}, var {t.var():.3e})", flush=True)
     assert final < 0.02 * t.var()
+
+
+M64 = (1 << 64) - 1
+
+
+def _mix64(z):
+    """splitmix64 finaliser on Python ints (the device sampler's hash)."""
+    z = (z + 0x9E3779B97F4A7C15) & M64
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & M64
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & M64
+    return z ^ (z >> 31)
+
+
+def _batch_indices(seed, step, n, B):
+    base = _mix64(seed ^ ((0x632BE59BD9B4E019 * step) & M64))
+    return np.array([_mix64((base + b) & M64) % n for b in range(B)], dtype=np.int64)
+
+
+def _pairs(n, seed=3):
+    rng = np.random.default_rng(seed)
+    x = rng.uniform(-1, 1, (n, 12)).astype(np.float32)
+    t = (np.sin(2 * x[:, 0]) + x[:, 1] * x[:, 2]).astype(np.float32)
+    w = rng.uniform(0.2, 2.0, n).astype(np.float32)
+    return x, t, w
+
+
+def _normalised_weights(wsel):
+    """The device's weight normalisation in its own float32 order: per block of 256 rows the
+    wave sums by xor butterflies, then the four waves in order; the block sums in order;
+    then w * (1 / total)."""
+    B = len(wsel)
+    nb = (B + 255) // 256
+    wpad = np.zeros(nb * 256, np.float32)
+    wpad[:B] = wsel
+    tot = np.float32(0.0)
+    for blk in wpad.reshape(nb, 4, 64):
+        waves = blk.copy()
+        for o in (32, 16, 8, 4, 2, 1):
+            waves = (waves + waves[:, np.arange(64) ^ o]).astype(np.float32)
+        bs = np.float32(0.0)
+        for q in range(4):
+            bs = np.float32(bs + waves[q, 0])
+        tot = np.float32(tot + bs)
+    return (wsel * (np.float32(1.0) / tot)).astype(np.float32)
+
+
+def _cosine_lr(lr, floor, it, steps):
+    """max(lr 0.5 (1 + cosf(pi it / steps)), floor) in float32 with the C library's cosf,
+    as the multi-step entry point evaluates it on the host."""
+    import ctypes
+    import ctypes.util
+    libm = ctypes.CDLL(ctypes.util.find_library("m"))
+    libm.cosf.restype = ctypes.c_float
+    libm.cosf.argtypes = [ctypes.c_float]
+    f = np.float32
+    arg = f(f(f(3.14159265358979) * f(it)) / f(steps))
+    v = f(f(f(lr) * f(0.5)) * f(f(1.0) + f(libm.cosf(arg))))
+    return float(max(v, f(floor)))
+
+
+def test_train_run_equals_stepwise_training_on_the_same_batches(lib):
+    """sm_mlp_train_run draws batch row b of step s at mix64(mix64(seed ^ K s) + b) mod n on
+    the device (restated here in Python), normalises the batch weights and follows a cosine
+    schedule; the same batches, weights and learning rates fed to the one-step entry point
+    give bit-identical parameters and loss."""
+    import torch
+    from sysmat import mlp
+    n, B, steps, lr, floor, seed = 5000, 256, 24, 2e-3, 2e-5, 77
+    x, t, w = _pairs(n)
+    xd, td, wd = (torch.from_numpy(a).cuda() for a in (x, t, w))
+    a = mlp.Mlp(seed=5)
+    b = mlp.Mlp(seed=5)
+    la = a.train_run(xd, td, wd, steps=steps, lr=lr, lr_floor=floor, seed=seed, batch=B, want_loss=True)
+    for it in range(steps):
+        idx = _batch_indices(seed, it + 1, n, B)
+        wb = _normalised_weights(w[idx])
+        ii = torch.from_numpy(idx).cuda()
+        lb = b.train_step(xd[ii].contiguous(), td[ii].contiguous(), torch.from_numpy(wb).cuda(),
+                          lr=_cosine_lr(lr, floor, it, steps), batch=B, want_loss=(it == steps - 1))
+    pa, pb = a.params.cpu().numpy(), b.params.cpu().numpy()
+    err = np.abs(pa - pb).max() / np.abs(pb).max()
+    print(f"train_run vs stepwise: max param diff {err:.2e} of the largest parameter; "
+          f"loss {float(la.item()):.8e} vs {float(lb.item()):.8e}", flush=True)
+    assert a.step == b.step == steps
+    assert np.array_equal(pa, pb)
+    assert float(la.item()) == float(lb.item())
+
+
+def test_train_run_is_deterministic_and_learns(lib):
+    import torch
+    from sysmat import mlp
+    n = 8192
+    x, t, w = _pairs(n, seed=4)
+    xd, td, wd = (torch.from_numpy(a).cuda() for a in (x, t, w))
+    nets = [mlp.Mlp(seed=11) for _ in range(2)]
+    for net in nets:
+        net.train_run(xd, td, wd, steps=400, lr=2e-3, lr_floor=2e-5, seed=9, batch=1024)
+    assert torch.equal(nets[0].params, nets[1].params)
+    final = float(((nets[0].forward(xd) - td) ** 2).mean().item())
+    print(f"train_run final mse {final:.3e} (var {t.var():.3e})", flush=True)
+    assert final < 0.02 * t.var()
+
+
+def test_members_trained_together_equal_members_trained_alone(lib):
+    import torch
+    from sysmat import mlp
+    n = 6000
+    x, t, w = _pairs(n, seed=6)
+    xd, td, wd = (torch.from_numpy(a).cuda() for a in (x, t, w))
+    together = [mlp.Mlp(seed=21 + m) for m in range(2)]
+    mlp.train_members(together, xd, td, wd, steps=40, lr=2e-3, lr_floor=2e-5, seeds=[3, 4], batch=512)
+    for m in range(2):
+        alone = mlp.Mlp(seed=21 + m)
+        alone.train_run(xd, td, wd, steps=40, lr=2e-3, lr_floor=2e-5, seed=3 + m, batch=512)
+        assert torch.equal(alone.params, together[m].params)
