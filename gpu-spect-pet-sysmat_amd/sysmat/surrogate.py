@@ -63,9 +63,9 @@ class TrainConfig:
     judge_rows: int = 32           # disjoint held-out rows (the judge)
     vox_per_row: int = 8192        # sampled voxels per training row (all if V is smaller)
     members: int = 2
-    steps: int = 3000
-    batch: int = 4096
-    lr: float = 2e-3
+    steps: int = 1500              # 1500 x 8192 rows: the samples of 3000 x 4096 in half the steps; on the
+    batch: int = 8192              # reduced C5 geometry the same fallback fraction (7.4 % of the bricks),
+    lr: float = 3e-3               # judge L1 0.35 % instead of 0.29 %, training 0.94 s instead of 1.58 s
     seed: int = 0
     phi_rel: float = 1e-4          # log floor relative to the largest training element
     tol: float = 0.02              # member disagreement (fraction of the element) that sends a tile to the exact kernel
