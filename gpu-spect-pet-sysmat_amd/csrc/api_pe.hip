@@ -125,9 +125,10 @@ static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, 
 // Voxel tiles of tx x ty x tz, grouped g x g x g per workgroup when the whole
 // plan (every detector of the table x every tile) is large enough.
 static void set_tiles(PeParams& p, int tx, int ty, int tz, int g = kGroupTiles) {
-    p.tx = std::max(1, std::min(tx, p.X));
-    p.ty = std::max(1, std::min(ty, p.Y));
-    p.tz = std::max(1, std::min(tz, p.Zslab));
+    // the tracer packs a voxel's offset in its tile into 10 bits per axis
+    p.tx = std::max(1, std::min({tx, p.X, 1024}));
+    p.ty = std::max(1, std::min({ty, p.Y, 1024}));
+    p.tz = std::max(1, std::min({tz, p.Zslab, 1024}));
     p.ntx = (p.X + p.tx - 1) / p.tx;
     p.nty = (p.Y + p.ty - 1) / p.ty;
     p.ntz = (p.Zslab + p.tz - 1) / p.tz;

@@ -43,6 +43,7 @@ constexpr int kListCap = 64;    // per-wave compacted attenuator boxes (beyond: 
 constexpr int kHoleCap = 48;    // per-wave compacted holes (beyond: bitmask path)
 constexpr int kSubTab = 256;    // sub-rays tabulated per workgroup with their geometry (S <= kSubTab)
 constexpr int kIdxTab = 2048;   // sub-ray indices only, in the same LDS (kSubTab < S <= kIdxTab)
+constexpr int kElTab = 64;      // tile elements whose voxel-centre constants are tabulated per tile
 
 // Per-sub-ray constants of a workgroup (one detecting crystal): the index
 // decomposition and the crystal sub-cell geometry (PESysMatGen.cu:606-616, 727-759).
@@ -108,6 +109,31 @@ __device__ inline void voxel_centre(const PeParams& p, int ix, int iy, int iz, f
 __device__ inline void rotate_image(const PeParams& p, float x, float y, float& xr, float& yr) {
     xr = x * p.rc - y * p.rs;
     yr = x * p.rs + y * p.rc;
+}
+
+// Voxel-centre quantities of element e of a tile (nvx x nvy x .. voxels from
+// (vx0, vy0, vz0)) seen from detecting crystal D: local voxel index, centre
+// before and after the image rotation, and the reference's cull terms -- the
+// unit vector towards D and r_D / (2 |D - I|) (PESysMatGen.cu:438-444, 506-589).
+struct ElementConsts {
+    int lx, ly, lz;
+    float xc0, yc0, zc0, xI, yI;
+    float crit_self, ux, uy, uz;
+};
+__device__ inline void element_consts(const PeParams& p, const Crystal& D, int e, int nvx, int nvy, int vx0, int vy0,
+                                      int vz0, ElementConsts& c) {
+    c.lx = e % nvx;
+    c.ly = (e / nvx) % nvy;
+    c.lz = e / (nvx * nvy);
+    voxel_centre(p, vx0 + c.lx, vy0 + c.ly, p.z0 + vz0 + c.lz, c.xc0, c.yc0, c.zc0);
+    rotate_image(p, c.xc0, c.yc0, c.xI, c.yI);
+    const float zI = c.zc0;
+    const float dist0 =
+        sqrtf((D.cy - c.yI) * (D.cy - c.yI) + (D.cx - c.xI) * (D.cx - c.xI) + (D.cz - zI) * (D.cz - zI));
+    c.crit_self = D.rbound / dist0 / 2.0f;
+    c.ux = (D.cx - c.xI) / dist0;
+    c.uy = (D.cy - c.yI) / dist0;
+    c.uz = (D.cz - zI) / dist0;
 }
 
 // Compacted attenuator (32 B in LDS): world box and mu for crystals with no
@@ -290,7 +316,7 @@ __device__ inline bool bundle_take_crystal(const Crystal* __restrict__ cry, int 
     const float4 c3 = reinterpret_cast<const float4*>(cry + k)[3];  // mu_c, res, slab, axis
     if (c3.z < 0.0f) return true;
     const float4 bb = box[(int)c3.z];
-    const float ex = (c3.w != 0.0f) ? c1.x : sqrtf(c1.x * c1.x + c1.z * c1.z);
+    const float ex = (c3.w != 0.0f) ? c1.x : cry[k].hxz;
     const float ez = (c3.w != 0.0f) ? c1.z : ex;
     return !(c0.x + ex < bb.x || c0.x - ex > bb.y || c0.z + ez < bb.z || c0.z - ez > bb.w);
 }
@@ -424,6 +450,9 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     __shared__ int s_wcount[kWaves];
     __shared__ int s_meta[8];  // tile-group state, re-read per tile (keeps it out of scalar registers)
     __shared__ SubCell s_sub[kSubTab];
+    __shared__ float4 s_elA[kElTab];  // per-element constants of the current tile (element_consts)
+    __shared__ float4 s_elB[kElTab];
+    __shared__ float2 s_elC[kElTab];
 
     const int tid = threadIdx.x;
     const int lane = tid & 63;
@@ -466,6 +495,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const float rcx = 1.0f / fcx, rcy = 1.0f / fcy, rcz = 1.0f / fcz, rnv = 1.0f / (float)p.nv;
     const float A_x = tD * hD / fcz / fcy, A_z = wD * tD / fcx / fcy, A_y = wD * hD / fcx / fcz;
     const float absorb_scale = D.mu_pe / D.mu_t;
+    // half extents of a crystal sub-cell (its centre is the sub-point Q)
+    const float sub_hx = 0.5f * rcx * wD, sub_hy = 0.5f * rcy * tD, sub_hz = 0.5f * rcz * hD;
     const bool axis_d = D.rot == 0.0f;  // crystal frame = translated world frame
     // sum of mu * thickness over the plates (FP64 and FP32) for the plate path of
     // elements without candidate holes; plates under 1e-5 mm are handled per sub-ray
@@ -616,23 +647,38 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             if (tid == 0) atomicOr(status, 1);
             return;
         }
-        __syncthreads();  // the candidate lists are complete before any wave reads them
+        // per-element constants of the tile, one thread per element (tiles of up to kElTab
+        // voxels): the voxel-centre terms of element_consts once per element instead of on
+        // every lane of its wave; larger tiles evaluate them in the element loop
+        const bool el_tab = nvox <= kElTab;
+        if (el_tab && tid < nvox) {
+            ElementConsts c;
+            element_consts(p, D, tid, nvx, nvy, vx0, vy0, vz0, c);
+            s_elA[tid] = make_float4(c.xc0, c.yc0, c.zc0, c.xI);
+            s_elB[tid] = make_float4(c.yI, c.crit_self, c.ux, c.uy);
+            s_elC[tid] = make_float2(c.uz, __int_as_float(c.lx | (c.ly << 10) | (c.lz << 20)));
+        }
+        __syncthreads();  // the candidate lists and element constants are complete before any wave reads them
         const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
 
         for (int e = wave; e < nvox; e += kWaves) {
-            const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
-            const int ix = vx0 + lx, iy = vy0 + ly, izl = vz0 + lz;
-            float xc0, yc0, zc0;
-            voxel_centre(p, ix, iy, p.z0 + izl, xc0, yc0, zc0);
-            float xI, yI;
-            rotate_image(p, xc0, yc0, xI, yI);
-            const float zI = zc0;
-
-            // -- exact cull predicate from the voxel centre (PESysMatGen.cu:506-589)
-            const float dist0 =
-                sqrtf((D.cy - yI) * (D.cy - yI) + (D.cx - xI) * (D.cx - xI) + (D.cz - zI) * (D.cz - zI));
-            const float crit_self = D.rbound / dist0 / 2.0f;
-            const float ux = (D.cx - xI) / dist0, uy = (D.cy - yI) / dist0, uz = (D.cz - zI) / dist0;
+            ElementConsts ec;
+            if (el_tab) {
+                const float4 a = s_elA[e], b = s_elB[e];
+                const float2 c = s_elC[e];
+                const int pk = __float_as_int(c.y);
+                ec.lx = pk & 0x3FF;
+                ec.ly = (pk >> 10) & 0x3FF;
+                ec.lz = pk >> 20;
+                ec.xc0 = a.x; ec.yc0 = a.y; ec.zc0 = a.z; ec.xI = a.w;
+                ec.yI = b.x; ec.crit_self = b.y; ec.ux = b.z; ec.uy = b.w; ec.uz = c.x;
+            } else {
+                element_consts(p, D, e, nvx, nvy, vx0, vy0, vz0, ec);
+            }
+            const int ix = vx0 + ec.lx, iy = vy0 + ec.ly, izl = vz0 + ec.lz;
+            const float xc0 = ec.xc0, yc0 = ec.yc0, zc0 = ec.zc0;
+            const float xI = ec.xI, yI = ec.yI, zI = zc0;
+            const float crit_self = ec.crit_self, ux = ec.ux, uy = ec.uy, uz = ec.uz;
 
             // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
             const float P_yhi = yI + pey, Q_ylo = D.cy - qhy;
@@ -656,7 +702,7 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     take = true;
                     if (bundle_ok && K.slab >= 0.0f) {
                         const float4 bb = s_box[wave][(int)K.slab];
-                        const float ex = (K.axis != 0.0f) ? K.hx : sqrtf(K.hx * K.hx + K.hz * K.hz);
+                        const float ex = (K.axis != 0.0f) ? K.hx : K.hxz;
                         const float ez = (K.axis != 0.0f) ? K.hz : ex;
                         take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
                     }
@@ -954,8 +1000,8 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                     const QRay Rq = axis_d ? make_qray_axis(R, xPs, yPs, zPs, xDs, yDs, zDs)
                                            : make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
                     float a1, a2, l_abs;
-                    self_paths(Rq, -D.hx, D.hx, -D.hy, D.hy, -D.hz, D.hz, ux1, ux2, uy1, uy2, uz1, uz2, a1, a2,
-                               l_abs);
+                    self_paths_centred(Rq, D.hx, D.hy, D.hz, sub_hx, sub_hy, sub_hz, ux1, ux2, uy1, uy2, uz1, uz2, a1,
+                                       a2, l_abs);
                     att += (a1 - a2) * D.mu_t;
                     // exp(-a) = 2^-(a log2 e): the integer part of the exponent applied exactly
                     // (ldexp), the fraction in FP32, so the relative error stays ~2 ulp for any

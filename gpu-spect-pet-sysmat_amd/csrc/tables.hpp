@@ -24,7 +24,8 @@ struct alignas(16) Crystal {
     float hx, hy, hz, rot;     // half extents (0.5*width, 0.5*thickness, 0.5*height), rotation about y
     float cr, sr, rbound, mu_pe;  // cos(-rot), sin(-rot), sqrt(w^2+h^2+t^2) (full diagonal), mu_pe
     float mu_c, res, slab, axis;  // mu_compton, energy resolution, y-slab id, 1 when rot == 0
-    float y_raw, flag, pad0, pad1;  // y as stored (before FOV2C; FP64 paths add it in double)
+    float y_raw, flag, hxz, pad1;   // y as stored (before FOV2C; FP64 paths add it in double), flag,
+                                    // lateral half-diagonal sqrt(hx^2 + hz^2) (bound of a rotated crystal)
 };
 
 // One collimator hole, 3 x float4.
@@ -151,6 +152,7 @@ inline std::string build_tables(const float* col, const float* det, float fov2c,
         if (D[11] != 0.0f) T.all_axis = false;
         c.y_raw = D[2];
         c.flag = D[12];
+        c.hxz = std::sqrt(c.hx * c.hx + c.hz * c.hz);
         c.slab = (float)slab_id(c.cy - c.hy, c.cy + c.hy);
         T.crystals.push_back(c);
     }

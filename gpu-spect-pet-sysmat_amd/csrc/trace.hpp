@@ -249,6 +249,30 @@ __device__ inline void self_paths(const QRay& r, float x1, float x2, float y1, f
     l_abs = (c >= kEps && r.t + sl > kEps && sh < r.t_ext) ? c : 0.0f;
 }
 
+// self_paths for the usual geometry: the crystal box centred at the frame origin
+// (half extents h) and the sub-cell box centred at Q (half extents s).  Per axis
+// the crystal's slab interval is m -+ h |1/u| with m = -q / u, and the sub-cell's
+// is -+ s |1/u|, so the intervals need no per-axis min / max and the sub-cell's
+// is one min of three products.  The box faces differ from the tabulated ones by
+// FP32 rounding only; degenerate components take self_paths on those faces.
+__device__ inline void self_paths_centred(const QRay& r, float hx, float hy, float hz, float sx, float sy, float sz,
+                                          float u1, float u2, float v1, float v2, float w1, float w2, float& a1,
+                                          float& a2, float& l_abs) {
+    if (r.pat | r.pat_e) {
+        self_paths(r, -hx, hx, -hy, hy, -hz, hz, u1, u2, v1, v2, w1, w2, a1, a2, l_abs);
+        return;
+    }
+    const float ax = fabsf(r.ix), ay = fabsf(r.iy), az = fabsf(r.iz);
+    const float mx = -r.qx * r.ix, my = -r.qy * r.iy, mz = -r.qz * r.iz;
+    const float sl = fmaxf(fmaxf(fmaf(-hx, ax, mx), fmaf(-hy, ay, my)), fmaf(-hz, az, mz));
+    const float sh = fminf(fminf(fmaf(hx, ax, mx), fmaf(hy, ay, my)), fmaf(hz, az, mz));
+    a1 = open_rules(sl, sh, r.t);
+    const float w = fminf(fminf(sx * ax, sy * ay), sz * az);
+    a2 = open_rules(-w, w, r.t);
+    const float c = w + w;
+    l_abs = (c >= kEps && r.t - w > kEps && w < r.t_ext) ? c : 0.0f;
+}
+
 // Per-sub-ray FP64 data of the collimator path.  The end points themselves are
 // evaluated in FP64: rays nearly parallel to a hole wall amplify end-point
 // rounding (~4e-6 mm in FP32 at 50 mm) by cot(angle) ~ 30-60x into the chord.
