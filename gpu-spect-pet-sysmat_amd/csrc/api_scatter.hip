@@ -853,6 +853,7 @@ struct SmSurrogate {
     DeviceBuf<SgRow> srow;     // [N]
     DeviceBuf<float> ftab;     // [N][kSgTab + 1] angular factor of far sites
     DeviceBuf<float4> ring;    // [N][n_keep][2] compact ranked sites for the ring prior
+    DeviceBuf<int> ring_cnt;   // [N] the ring prior's loop bound
     SgParams q{};
     int n_near = 0, n_ring = 0;
 };
@@ -971,7 +972,9 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
         HIP_TRY(hipMemset(S->keep_cnt.ptr, 0, sizeof(int) * n));
     }
     if ((rc = S->ring.alloc((size_t)n * n_keep * 2)) < 0) return fail(rc);
-    HIP_TRY(launch_surrogate_ring_table(S->keep.ptr, S->keep_cnt.ptr, n, n_keep, S->ring.ptr, nullptr));
+    if ((rc = S->ring_cnt.alloc((size_t)n)) < 0) return fail(rc);
+    HIP_TRY(launch_surrogate_ring_table(S->keep.ptr, S->keep_cnt.ptr, n, n_keep, n_near, S->ring.ptr,
+                                        S->ring_cnt.ptr, nullptr));
     HIP_TRY(hipDeviceSynchronize());
     // per-row feature constants
     float bmin[3] = {1e30f, 1e30f, 1e30f}, bmax[3] = {-1e30f, -1e30f, -1e30f};
@@ -1029,6 +1032,7 @@ int sm_surrogate_create(const float* det, const float* img, const float* phy, in
     q.n_tiles = (long long)((b.X + 3) / 4) * ((b.Y + 3) / 4) * ((b.Z + 3) / 4);
     q.ftab = S->ftab.ptr;
     q.ring = S->ring.ptr;
+    q.ring_cnt = S->ring_cnt.ptr;
     *out = S;
     return SM_OK;
 }

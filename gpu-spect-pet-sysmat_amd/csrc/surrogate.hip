@@ -24,8 +24,9 @@ __device__ inline void sg_position(const SgParams& q, long long v, float& xI, fl
 
 // Ring prior over sites j = j0, j0 + 2, ... < cnt of the row's ranked list (rt:
 // the row's two float4 per site): each site's weight re-evaluated at this
-// voxel's scatter angle.  Unrolled so several PE gathers are in flight; a
-// zero-weight site adds exactly zero (its load is skipped).  The prior is the
+// voxel's scatter angle.  Unrolled so several PE gathers are in flight (no
+// branch: every factor is finite and >= 0, so a zero-weight site adds +0, as if
+// skipped).  The prior is the
 // even-site sum plus the odd-site sum, so the fused kernel can give the two
 // halves to the two half-waves.
 constexpr int kRingUnroll = 4;
@@ -39,8 +40,30 @@ __device__ inline float sg_ring(const SgParams& q, const float4* __restrict__ rt
         const float wg = a.w;
         const float ax = a.x - xI, ay = a.y - yI, az = a.z - zI;
         const float c = (ax * b.x + ay * b.y + az * b.z) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
-        const float pv = wg != 0.0f ? pe[(long long)__float_as_int(b.w) * q.V + v] : 0.0f;
-        ring += wg != 0.0f ? pv * wg * sg_angular(tab, c) : 0.0f;
+        const float pv = pe[(long long)__float_as_int(b.w) * q.V + v];
+        ring += pv * wg * sg_angular(tab, c);
+    }
+    return ring;
+}
+
+// sg_ring for the fused kernel: the half-wave `half` takes the sites of its parity,
+// in a loop over site pairs whose trip count is wave-uniform (so it unrolls and the
+// gathers of several sites are in flight); the lane past the list's end adds +0.
+// Same sums, in the same order, as sg_ring(q, rt, n_near + half, ...).
+__device__ inline float sg_ring_half(const SgParams& q, const float4* __restrict__ rt, int half, int cnt,
+                                     const float* tab, const float* __restrict__ pe, long long v, float xI, float yI,
+                                     float zI) {
+    float ring = 0.0f;
+#pragma unroll kRingUnroll
+    for (int jp = q.n_near; jp < cnt; jp += 2) {
+        const bool ok = jp + half < cnt;
+        const int j = ok ? jp + half : jp;
+        const float4 a = rt[2 * j], b = rt[2 * j + 1];
+        const float ax = a.x - xI, ay = a.y - yI, az = a.z - zI;
+        const float c = (ax * b.x + ay * b.y + az * b.z) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
+        const float pv = pe[(long long)__float_as_int(b.w) * q.V + v];
+        const float t = pv * a.w * sg_angular(tab, c);
+        ring += ok ? t : 0.0f;
     }
     return ring;
 }
@@ -60,12 +83,30 @@ __global__ void surrogate_ring_table_kernel(const ScPair* __restrict__ keep, con
     ring[2 * i + 1] = b;
 }
 
-hipError_t launch_surrogate_ring_table(const ScPair* keep, const int* keep_cnt, int n_det, int n_keep, float4* ring,
-                                      hipStream_t s) {
+// Per row: 1 + rank of the last site (rank >= n_near) with a non-zero ring weight.
+// Sites past it add +0 to the prior, so stopping there changes no sum.
+__global__ void surrogate_ring_count_kernel(const ScPair* __restrict__ keep, const int* __restrict__ keep_cnt,
+                                            int n_det, int n_keep, int n_near, int* __restrict__ ring_cnt) {
+    const int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n_det) return;
+    int c = n_near;
+    for (int j = keep_cnt[r] - 1; j >= n_near; --j) {
+        if (keep[(long long)r * n_keep + j].hy1 != 0.0f) {
+            c = j + 1;
+            break;
+        }
+    }
+    ring_cnt[r] = c;
+}
+
+hipError_t launch_surrogate_ring_table(const ScPair* keep, const int* keep_cnt, int n_det, int n_keep, int n_near,
+                                      float4* ring, int* ring_cnt, hipStream_t s) {
     const long long n = (long long)n_det * n_keep;
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(surrogate_ring_table_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, keep, keep_cnt,
                        n_det, n_keep, ring);
+    hipLaunchKernelGGL(surrogate_ring_count_kernel, dim3((unsigned)((n_det + 255) / 256)), dim3(256), 0, s, keep,
+                       keep_cnt, n_det, n_keep, n_near, ring_cnt);
     return hipGetLastError();
 }
 
@@ -103,7 +144,7 @@ __global__ void __launch_bounds__(256) surrogate_features_kernel(SgParams q, con
     const int r = q.det_begin + rr;
     const long long v = vox ? (long long)vox[j] : (long long)j;
     const float4* kr = q.ring + 2LL * r * q.n_keep;
-    const int cnt = keep_cnt[r];
+    const int cnt = q.ring_cnt[r];
     const float* tab = q.ftab + (long long)r * (kSgTab + 1);
     float xI, yI, zI;
     sg_position(q, v, xI, yI, zI);
@@ -115,18 +156,26 @@ __global__ void __launch_bounds__(256) surrogate_features_kernel(SgParams q, con
     for (int k = 0; k < kSgFeatures; ++k) x[i * kSgFeatures + k] = f[k];
 }
 
-// Voxel i (0..31) of wave chunk `ch` of a row: half (ch & 1) of the 4 x 4 x 4
-// brick ch / 2 (x fastest, the half holds z-layers 2 (ch & 1) and 2 (ch & 1) + 1).
+// Voxel i (0..31) of wave chunk `ch` of a row: the run of 32 consecutive voxels
+// (row order, x fastest) from 32 ch, so each ring-prior site's PE gather of a
+// wave reads whole cache lines.
 __device__ inline long long sg_voxel(const SgParams& q, long long ch, int i, bool& ok) {
-    const int nbx = (q.X + 3) >> 2, nby = (q.Y + 3) >> 2;
-    const long long b = ch >> 1;
-    const int bx = (int)(b % nbx), b2 = (int)(b / nbx), by = b2 % nby, bz = b2 / nby;
-    const int ix = 4 * bx + (i & 3), iy = 4 * by + ((i >> 2) & 3), iz = 4 * bz + 2 * (int)(ch & 1) + (i >> 4);
-    ok = ix < q.X && iy < q.Y && iz < q.Z;
-    return ((long long)iz * q.Y + iy) * q.X + ix;
+    const long long v = 32 * ch + i;
+    ok = v < q.V;
+    return v;
 }
 
-// One wave = half a 4 x 4 x 4 voxel brick (32 voxels) of one detector row.
+// 4 x 4 x 4 brick (exact-fallback tile) of voxel v.
+__device__ inline long long sg_brick(const SgParams& q, long long v) {
+    const long long XY = (long long)q.X * q.Y;
+    const int iz = (int)(v / XY);
+    const int rem = (int)(v % XY);
+    const int iy = rem / q.X, ix = rem % q.X;
+    const int nbx = (q.X + 3) >> 2, nby = (q.Y + 3) >> 2;
+    return ((long long)(iz >> 2) * nby + (iy >> 2)) * nbx + (ix >> 2);
+}
+
+// One wave = a run of 32 consecutive voxels of one detector row.
 // Features go through LDS into the MFMA B-operand layout; every member runs
 // with register-resident activations and weight tiles streamed from L2 (each
 // wave on its own: sharing weight slabs through LDS cost two workgroup barriers
@@ -141,8 +190,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
                                                               const float* __restrict__ pe, float* __restrict__ dst,
                                                               uint8_t* __restrict__ flags) {
     __shared__ float s_f[4][32][kSgFeatures + 1];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4, j16 = lane & 15;
-    const long long chunks = 2 * q.n_tiles;
+    __shared__ float s_tab[4][kSgTab + 1];  // the wave's row angular table
+    // the wave index through readfirstlane: what derives from it is known to be wave-uniform
+    const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int g = lane >> 4, j16 = lane & 15;
+    const long long chunks = (q.V + 31) / 32;
     const long long gw = (long long)blockIdx.x * 4 + w;
     const int rr = (int)(gw / chunks);
     const long long ch = gw % chunks;
@@ -151,6 +203,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
     const SgRow R = rows[r];
     const bool use = valid_wave && R.use != 0.0f;
     {
+        const float* tab = q.ftab + (long long)r * (kSgTab + 1);
+        for (int i = lane; i <= kSgTab; i += 64) s_tab[w][i] = tab[i];
+        __syncthreads();  // the rows' angular tables are in LDS
         // lanes e and e + 32 share element e: even and odd ring sites
         const int e = lane & 31, half = lane >> 5;
         bool ok;
@@ -159,8 +214,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
         float xI = 0.0f, yI = 0.0f, zI = 0.0f, ring = 0.0f;
         if (live) {
             sg_position(q, v, xI, yI, zI);
-            ring = sg_ring(q, q.ring + 2LL * r * q.n_keep, q.n_near + half, keep_cnt[r],
-                           q.ftab + (long long)r * (kSgTab + 1), pe, v, xI, yI, zI);
+            ring = sg_ring_half(q, q.ring + 2LL * r * q.n_keep, half, q.ring_cnt[r], s_tab[w], pe, v, xI, yI, zI);
         }
         const float other = __shfl_xor(ring, 32, 64);
         if (half == 0) {
@@ -196,7 +250,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
             far_max[t] = fmaxf(far_max[t], fr);
         }
     }
-    bool flag = false;
     if (g == 0 && use) {
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
@@ -205,7 +258,10 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
             if (!ok) continue;
             float* o = dst + rr * q.row_stride + v;
             const float s_hat = *o + far_sum[t] / (float)q.n_members;
-            if (q.n_members > 1 && s_hat > q.floor_abs && (far_max[t] - far_min[t]) > q.tol * s_hat) flag = true;
+            // members disagree: the voxel's brick goes to the exact kernel (lanes of one
+            // brick store the same byte)
+            if (q.n_members > 1 && s_hat > q.floor_abs && (far_max[t] - far_min[t]) > q.tol * s_hat && flags)
+                flags[(long long)rr * q.n_tiles + sg_brick(q, v)] = 1;
             *o = s_hat;
         }
     } else if (g == 0 && valid_wave) {
@@ -216,8 +272,6 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) s
             if (ok) dst[rr * q.row_stride + v] = 0.0f;
         }
     }
-    const unsigned long long any = __ballot(flag);
-    if (lane == 0 && any && flags) flags[(long long)rr * q.n_tiles + (ch >> 1)] = 1;
 }
 
 hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
@@ -232,7 +286,7 @@ hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const
 
 hipError_t launch_surrogate_apply(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                   const uint8_t* packed, const float* pe, float* dst, uint8_t* flags, hipStream_t s) {
-    const long long per_row = 2 * q.n_tiles;  // waves per detector row: two per brick
+    const long long per_row = (q.V + 31) / 32;  // waves per detector row: runs of 32 voxels
     if (per_row <= 0 || q.det_count <= 0) return hipSuccess;
     // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
     const long long max_waves = ((1LL << 32) / 256 - 1) * 4;

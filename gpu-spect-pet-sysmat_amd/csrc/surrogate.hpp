@@ -40,13 +40,15 @@ struct SgParams {
     const float* ftab;          // [n_det][kSgTab + 1] angular factor tables
     const float4* ring;         // [n_det][n_keep][2] ranked sites for the ring prior: (x, y, z, weight),
                                 // (unit vector to the detector, site index bits); zero past the row's count
+    const int* ring_cnt;        // [n_det] 1 + rank of the row's last site with a non-zero ring weight
+                                // (n_near when none): the ring prior's loop bound
 };
 
 hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                      const float* pe, const float* near, long long near_stride, const int* vox,
                                      int n_vox, float* x, hipStream_t s);
-hipError_t launch_surrogate_ring_table(const ScPair* keep, const int* keep_cnt, int n_det, int n_keep, float4* ring,
-                                      hipStream_t s);
+hipError_t launch_surrogate_ring_table(const ScPair* keep, const int* keep_cnt, int n_det, int n_keep, int n_near,
+                                      float4* ring, int* ring_cnt, hipStream_t s);
 hipError_t launch_surrogate_apply(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                   const uint8_t* packed, const float* pe, float* dst, uint8_t* flags, hipStream_t s);
 
