@@ -110,6 +110,15 @@ static int fill_params(const float* img, const SmPeOptions& o, const Tables& T, 
     if (S > (1 << 24)) return set_error(SM_E_ARG, "too many sub-rays per element");
     p.S = (int)S;
     p.rnv64 = 1.0 / (double)p.nv;
+    p.hX64 = (double)p.X / 2.0;
+    p.hY64 = (double)p.Y / 2.0;
+    p.hZ64 = (double)p.Zfull / 2.0;
+    p.wx64 = (double)p.wx;
+    p.wy64 = (double)p.wy;
+    p.wz64 = (double)p.wz;
+    p.sx64 = (double)p.sx;
+    p.sy64 = (double)p.sy;
+    p.sz64 = (double)p.sz;
     p.rcx64 = 1.0 / (double)p.cx;
     p.rcy64 = 1.0 / (double)p.cy;
     p.rcz64 = 1.0 / (double)p.cz;
@@ -458,6 +467,7 @@ int sm_PESysMatGen(const float* col, const float* det, const float* img, float* 
         // Img[30] unset or short of the slab: the candidate lists follow the
         // voxel positions, so cull them again for the reference's positions
         P->base.Zfull = fullZ;
+        P->base.hZ64 = (double)fullZ / 2.0;  // the FP64 centre of the collimator path follows it
         if ((rc = build_lists(*P)) < 0) {
             sm_pe_plan_destroy(plan);
             return rc;

@@ -820,9 +820,9 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
             double acc = 0.0;
             if (!element_blocked) {
                 // FP64 voxel centre (before rotation) for the collimator path
-                const double xc64 = ((double)ix - (double)p.X / 2.0 + 0.5) * (double)p.wx + (double)p.sx;
-                const double yc64 = ((double)iy - (double)p.Y / 2.0 + 0.5) * (double)p.wy + (double)p.sy;
-                const double zc64 = ((double)(p.z0 + izl) - (double)p.Zfull / 2.0 + 0.5) * (double)p.wz + (double)p.sz;
+                const double xc64 = ((double)ix - p.hX64 + 0.5) * p.wx64 + p.sx64;
+                const double yc64 = ((double)iy - p.hY64 + 0.5) * p.wy64 + p.sy64;
+                const double zc64 = ((double)(p.z0 + izl) - p.hZ64 + 0.5) * p.wz64 + p.sz64;
                 const bool att_fast = n_att <= kListCap;
                 const bool hol_fast = n_hol <= kHoleCap;
                 for (int base = 0; base < S; base += 64) {
@@ -858,72 +858,20 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                         }
                         sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
                     }
-                    float xP = xI, yP = yI, zP = zI;
-                    if (p.nv > 1) {
-                        const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
-                        const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
-                        const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
-                        rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
-                        zP = zc0 + oz;
-                    }
-                    const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
-                    const float yPs = yP - D.cy;
-                    const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
-                    // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
-                    const float xDs = SC.xDs, yDs = SC.yDs, zDs = SC.zDs;
-                    const float xQ = SC.xQ, yQ = SC.yQ, zQ = SC.zQ;
-
-                    const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
-                    const float dist = R.t;
-                    // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
-                    const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
-                    const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
-                    const float fy = A_y * fabsf(R.dy);
-                    const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
-
-                    bool blocked = false;
-                    // plate miss => blocked (PESysMatGen.cu:645-659)
-                    for (unsigned mm = plate_test; mm; mm &= mm - 1) {
-                        const int m = __builtin_ctz(mm);
-                        const Plate PL = pla[m];
-                        const float c =
-                            (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
-                                         : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t, R.pat,
-                                                         PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
-                        if (c < 0.1f) blocked = true;
-                    }
                     float att = 0.0f;
                     double att64 = 0.0;
-                    // collimator layers: plate length minus hole chords (PESysMatGen.cu:661-685)
-                    if (layer_holes == 0) {
-                        // plate path thick * |PQ| / dy = thick (1 + q), q = r^2 / (1 + sqrt(1 + r^2)),
-                        // r = lateral / dy: mu * thick summed in FP64 and only the small q term
-                        // rounded in FP32, so the exponent of strongly attenuated elements (tens
-                        // of mean free paths) keeps ~1e-7 absolute accuracy.  dy <= 0 gives a
-                        // negative length, which the reference does not add.
-                        if (R.dy > 0.0f) {
-                            const float idy = frcp(R.dy);
-                            const float r2 = (R.dx * R.dx + R.dz * R.dz) * (idy * idy);
-                            const float qv = r2 * frcp(1.0f + fsqrt(1.0f + r2));
-                            att64 += plate_mt + (double)(plate_mtf * qv);
-                            if (thin_plates) {  // plates under 1e-5 mm count only along very oblique paths
-                                for (int m = 0; m < p.n_layers; ++m) {
-                                    const Plate PL = pla[m];
-                                    if (PL.thick < 0.00001f && PL.thick * (1.0f + qv) >= 0.00001f)
-                                        att64 += (double)PL.mu_t * (double)PL.thick +
-                                                 (double)((PL.mu_t * PL.thick) * qv);
-                                }
-                            }
-                        }
-                    } else {
+                    // collimator layers, plate length minus hole chords (PESysMatGen.cu:661-685),
+                    // when some hole is a candidate: first, because this FP64 path needs only the
+                    // sub-ray's indices, so its registers are free again before the FP32 ray is built
+                    if (layer_holes != 0) {
                         // end points in FP64 with the same formulas (quotients by the
                         // sampling counts through their reciprocals, correctly rounded)
                         double px = xc64, py = yc64, pz = zc64;
                         if (p.nv > 1) {
                             const double fnv = (double)p.nv;
-                            px += (div_rcp((double)va + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wx;
-                            py += (div_rcp((double)vb + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wy;
-                            pz += (div_rcp((double)vc + 0.5, fnv, p.rnv64) - 0.5) * (double)p.wz;
+                            px += (div_rcp((double)va + 0.5, fnv, p.rnv64) - 0.5) * p.wx64;
+                            py += (div_rcp((double)vb + 0.5, fnv, p.rnv64) - 0.5) * p.wy64;
+                            pz += (div_rcp((double)vc + 0.5, fnv, p.rnv64) - 0.5) * p.wz64;
                         }
                         const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
                         const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
@@ -962,6 +910,62 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
                             const double in_plate = div_rcp((double)PL.thick, R64.ky, R64.iky);
                             if ((in_plate - holes_len) >= (double)0.00001f)
                                 att64 += (double)PL.mu_t * (in_plate - holes_len);
+                        }
+                    }
+                    float xP = xI, yP = yI, zP = zI;
+                    if (p.nv > 1) {
+                        const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
+                        const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
+                        const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
+                        rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
+                        zP = zc0 + oz;
+                    }
+                    const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
+                    const float yPs = yP - D.cy;
+                    const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
+                    // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
+                    const float xDs = SC.xDs, yDs = SC.yDs, zDs = SC.zDs;
+                    const float xQ = SC.xQ, yQ = SC.yQ, zQ = SC.zQ;
+
+                    const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
+                    const float dist = R.t;
+                    // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
+                    const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
+                    const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
+                    const float fy = A_y * fabsf(R.dy);
+                    const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
+
+                    bool blocked = false;
+                    // plate miss => blocked (PESysMatGen.cu:645-659)
+                    for (unsigned mm = plate_test; mm; mm &= mm - 1) {
+                        const int m = __builtin_ctz(mm);
+                        const Plate PL = pla[m];
+                        const float c =
+                            (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
+                                         : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t, R.pat,
+                                                         PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
+                        if (c < 0.1f) blocked = true;
+                    }
+                    // collimator layers without candidate holes (PESysMatGen.cu:661-685)
+                    if (layer_holes == 0) {
+                        // plate path thick * |PQ| / dy = thick (1 + q), q = r^2 / (1 + sqrt(1 + r^2)),
+                        // r = lateral / dy: mu * thick summed in FP64 and only the small q term
+                        // rounded in FP32, so the exponent of strongly attenuated elements (tens
+                        // of mean free paths) keeps ~1e-7 absolute accuracy.  dy <= 0 gives a
+                        // negative length, which the reference does not add.
+                        if (R.dy > 0.0f) {
+                            const float idy = frcp(R.dy);
+                            const float r2 = (R.dx * R.dx + R.dz * R.dz) * (idy * idy);
+                            const float qv = r2 * frcp(1.0f + fsqrt(1.0f + r2));
+                            att64 += plate_mt + (double)(plate_mtf * qv);
+                            if (thin_plates) {  // plates under 1e-5 mm count only along very oblique paths
+                                for (int m = 0; m < p.n_layers; ++m) {
+                                    const Plate PL = pla[m];
+                                    if (PL.thick < 0.00001f && PL.thick * (1.0f + qv) >= 0.00001f)
+                                        att64 += (double)PL.mu_t * (double)PL.thick +
+                                                 (double)((PL.mu_t * PL.thick) * qv);
+                                }
+                            }
                         }
                     }
                     // other crystals on the way (PESysMatGen.cu:687-725)

@@ -39,6 +39,8 @@ struct PeParams {
     float inv4pi;
     double rc64, rs64;         // image rotation in double (FP64 end points of the collimator path)
     double rnv64, rcx64, rcy64, rcz64;  // 1 / nv, 1 / cx, 1 / cy, 1 / cz (correctly rounded)
+    double hX64, hY64, hZ64;   // X / 2, Y / 2, Zfull / 2 (FP64 voxel centres of the collimator path)
+    double wx64, wy64, wz64, sx64, sy64, sz64;  // voxel size and image shift in FP64
     float fov2c;               // Img[11]
 };
 
