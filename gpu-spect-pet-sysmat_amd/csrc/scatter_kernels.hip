@@ -589,6 +589,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
         // scatter angles outside [th_lo, th_hi] fail the energy-window test of the site term;
         // 2 mrad covers the FP32 rounding of the bound below (at most ~5e-4 rad near cos = 1)
         const float thl = (float)R.th_lo - 2e-3f, thh = (float)R.th_hi + 2e-3f;
+        // the same test in cosines (no acos / asin per site) where the band edges lie in
+        // (0, pi): cos is strictly decreasing there
+        const bool cos_test = thl < 3.1f && thh > 0.05f;
+        const bool hi_open = thh >= 3.14159265f, lo_open = thl <= 0.0f;
+        const float chh = cosf(fminf(thh, 3.14159265f)), cll = cosf(fmaxf(thl, 0.0f));
         const KnCoef kc = kn_coef(p.e0);
         const int nch = site_chunks(p), csz = site_chunk_size(p);
         int cap = (p.site_cap > 0) ? p.site_cap : 0x7fffffff;
@@ -609,9 +614,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
                     const float ila = __builtin_amdgcn_rsqf(la2);
                     const float c0 =
                         fminf(1.0f, fmaxf(-1.0f, (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * ila));
-                    const float th0 = acosf(c0);
-                    const float dth = asinf(fminf(1.0f, trad * ila * 1.0001f));
-                    if (th0 - dth > thh || th0 + dth < thl) continue;
+                    const float s1 = fminf(1.0f, trad * ila * 1.0001f);  // sin dth
+                    if (cos_test) {
+                        // th0 - dth > thh  <=>  th0 > dth and cos(th0 - dth) < cos thh;
+                        // th0 + dth < thl  <=>  th0 + dth < pi and cos(th0 + dth) > cos thl
+                        const float c1 = fsqrt(fmaxf(0.0f, 1.0f - s1 * s1));  // cos dth
+                        const float s0 = fsqrt(fmaxf(0.0f, 1.0f - c0 * c0));  // sin th0
+                        const bool above = !hi_open && c0 < c1 && c0 * c1 + s0 * s1 < chh;
+                        const bool below = !lo_open && s0 * c1 + c0 * s1 > 0.0f && c0 * c1 - s0 * s1 > cll;
+                        if (above || below) continue;
+                    } else {
+                        const float th0 = acosf(c0);
+                        const float dth = asinf(s1);
+                        if (th0 - dth > thh || th0 + dth < thl) continue;
+                    }
                 }
                 if (!active) continue;
                 const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
