@@ -41,7 +41,10 @@ __device__ inline float sg_ring(const SgParams& q, const float4* __restrict__ rt
         const float ax = a.x - xI, ay = a.y - yI, az = a.z - zI;
         const float c = (ax * b.x + ay * b.y + az * b.z) * __builtin_amdgcn_rsqf(ax * ax + ay * ay + az * az);
         const float pv = pe[(long long)__float_as_int(b.w) * q.V + v];
-        ring += pv * wg * sg_angular(tab, c);
+        // the select keeps the product rounded before the sum (no FMA contraction), as in
+        // sg_ring_half
+        const float t = pv * wg * sg_angular(tab, c);
+        ring += wg != 0.0f ? t : 0.0f;
     }
     return ring;
 }
@@ -125,7 +128,9 @@ __device__ inline void sg_finish(const SgParams& q, const SgRow& R, int r, long 
     f[7] = 0.1f * __logf(ring * inv_s + 1e-6f);
     f[8] = 0.05f * __logf(pe[(long long)r * q.V + v] / q.pe_scale + 1e-9f);
     const float dx = R.cx - xI, dy = R.cy - yI, dz = R.cz - zI;
-    const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+    // explicit FMAs: the same rounding in every kernel this is inlined into (left to
+    // contraction, the features and the fused apply kernels rounded it differently)
+    const float d = sqrtf(__builtin_fmaf(dz, dz, __builtin_fmaf(dy, dy, dx * dx)));
     f[9] = dx / d;
     f[10] = dz / d;
     f[11] = __logf(d / q.d_ref);

@@ -98,3 +98,42 @@ def test_surrogate_judge_and_ablation(lib):
     assert abs(j["total_rel_err"]) <= 0.01
     assert j["l1_rel_err"] < j["near_only_l1_rel_err"]
     assert set(s.train_rows).isdisjoint(set(s.judge_rows))
+
+
+def test_fused_apply_equals_features_plus_generic_inference(lib):
+    """The fused apply kernel (ring prior split over half-waves, voxel runs of 32) computes
+    the same features as the training path: one member, no fallback, against the features
+    kernel + the generic inference kernel + the far-field formula in FP64.  A feature that
+    differed would change a bf16 network input (~1e-3 of the far field), far above the
+    tolerance, which covers the kernel's FP32 fast exp (exponent ~ -23: a few 1e-6 of
+    exp(.) = far + phi) and the FP32 sum with the near field."""
+    import torch
+    from sysmat import mlp
+    from sysmat.pe import PePlan
+    from sysmat.surrogate import ScatterSurrogate, TrainConfig
+    g = c3_small()
+    N, V = g.n_det, g.n_vox
+    pe = torch.empty((N, V), dtype=torch.float32, device="cuda")
+    plan = PePlan(g, voxel_sub=1, sub=(2, 2, 2))
+    plan.launch(0, N, pe)
+    plan.sync()
+    s = ScatterSurrogate(g, n_near=8, n_ring=24)
+    s.train(pe, TrainConfig(train_rows=32, judge_rows=4, vox_per_row=2048, members=2, steps=200, batch=2048, seed=1))
+    nm = s.norm
+    checked = 0
+    for r in [int(x) for x in s.judge_rows[:3]]:
+        near = torch.zeros((1, V), dtype=torch.float32, device="cuda")
+        s.near(pe, r, 1, near)
+        out = torch.zeros_like(near)
+        s.apply(pe, r, 1, out, fallback=False, n_members=1)
+        x = s.features(pe, near, r, 1)
+        y = mlp.infer(s.packed, 1, x)[0].double()
+        e = torch.exp(y * nm["sig_y"] + nm["mu_y"])
+        far = torch.clamp(e - nm["phi"], min=0.0)
+        want = (near[0].double() + far).cpu().numpy()
+        got = out[0].double().cpu().numpy()
+        assert got.max() > 0 and far.max().item() > 0
+        allowed = 8e-6 * e.cpu().numpy() + 2e-7 * np.abs(want)
+        assert np.all(np.abs(got - want) <= allowed), float(np.max(np.abs(got - want) / allowed))
+        checked += 1
+    assert checked == 3
