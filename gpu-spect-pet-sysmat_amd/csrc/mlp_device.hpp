@@ -45,18 +45,23 @@ __device__ __forceinline__ void act_pack(const f32x4 (&acc)[2][2], bf16x8 (&out)
     }
 }
 
-// z = W in + bias for output tile pair kbo of a 256 -> 256 layer.
+// z = W in + bias for output tile pair kbo of a 256 -> 256 layer; its 16 weight tiles
+// are requested together before the first MFMA, so their L2 latencies overlap.
 __device__ __forceinline__ void tile_pair(const uint8_t* __restrict__ W, const float* __restrict__ bias,
                                           const bf16x8 (&in)[8][2], int kbo, int lane, f32x4 (&acc)[2][2]) {
+    bf16x8 a0[8], a1[8];
+#pragma unroll
+    for (int kbi = 0; kbi < 8; ++kbi) {
+        a0[kbi] = ld_tile(W, (2 * kbo) * 8 + kbi, lane);
+        a1[kbi] = ld_tile(W, (2 * kbo + 1) * 8 + kbi, lane);
+    }
     bias_acc(bias, kbo, lane >> 4, acc);
 #pragma unroll
     for (int kbi = 0; kbi < 8; ++kbi) {
-        const bf16x8 a0 = ld_tile(W, (2 * kbo) * 8 + kbi, lane);
-        const bf16x8 a1 = ld_tile(W, (2 * kbo + 1) * 8 + kbi, lane);
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
-            acc[0][t] = mfma16(a0, in[kbi][t], acc[0][t]);
-            acc[1][t] = mfma16(a1, in[kbi][t], acc[1][t]);
+            acc[0][t] = mfma16(a0[kbi], in[kbi][t], acc[0][t]);
+            acc[1][t] = mfma16(a1[kbi], in[kbi][t], acc[1][t]);
         }
     }
 }

@@ -186,9 +186,11 @@ __device__ inline long long sg_brick(const SgParams& q, long long v) {
 // wave on its own: sharing weight slabs through LDS cost two workgroup barriers
 // per slab, more than the L2 reads it saved).  Disagreeing members flag the
 // brick (tile) for the exact kernel, which prunes scatter sites per brick.
-// Three waves per SIMD (168 VGPRs, a few spilled) hide the ring prior's PE
-// gathers behind the other waves' MFMAs better than two without spills.
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3))) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
+// Two waves per SIMD (230 VGPRs, no spill): each output tile pair's 16 weight
+// tiles are loaded before its MFMAs (tile_pair), which needs the registers of a
+// third wave and hides more L2 latency than that wave did (-4 % on a 64^3 image,
+// -7 % on 128^3, identical output).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
                                                               const ScPair* __restrict__ keep,
                                                               const int* __restrict__ keep_cnt,
                                                               const uint8_t* __restrict__ packed,
