@@ -185,7 +185,9 @@ static int build_geom(const float* col, const float* det, const float* img, cons
     p.e0 = phy[7];
     if (!(p.e0 > 0.0)) return set_error(SM_E_ARG, "photopeak energy Phys[7] must be positive");
     p.kn_total = kn_total_ref(p.e0);
+    p.ikn_total = 1.0 / p.kn_total;
     p.h = (double)0.01f;
+    p.ihf = (float)(1.0 / p.h);
     {
         std::vector<double2> tab(kKnSteps + 1);
         for (int n = 0; n <= kKnSteps; ++n) {

@@ -75,7 +75,9 @@ struct ScParams {
     double rc, rs;            // image rotation (Img[20] * Img[7])
     double e0;                // photopeak energy (Phys[7])
     double kn_total;          // KN Riemann sum over [0, pi] (reference normalisation)
+    double ikn_total;         // 1 / kn_total
     double h;                 // KN step (0.01f)
+    float ihf;                // 1 / h in FP32 (step count outside the guard band)
     const double2* kn_tab;    // (cos n h, sin n h), n = 0 .. kKnSteps
     int n_det;
     int det_begin, det_count; // detector rows of this launch

@@ -394,7 +394,8 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
 __device__ inline float window_probability(double e, double res, double lo, double hi) {
     // 0.5 (erf(z2) - erf(z1)), z = (bound - E') / (sqrt(2) sigma); tails via erfc
     const double s2 = res / (double)2.35482f * e * 1.4142135623730951;
-    const float z1 = (float)((lo - e) / s2), z2 = (float)((hi - e) / s2);
+    const double is2 = 1.0 / s2;  // one division for both bounds
+    const float z1 = (float)((lo - e) * is2), z2 = (float)((hi - e) * is2);
     if (z1 >= 0.0f) return 0.5f * (erfcf(z1) - erfcf(z2));
     if (z2 <= 0.0f) return 0.5f * (erfcf(-z2) - erfcf(-z1));
     return 0.5f * (erff(z2) - erff(z1));
@@ -459,7 +460,8 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     const double ima = 1.0 / ma;
     double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
     c = fmin(1.0, fmax(-1.0, c));
-    const double es = e0 / (1.0 + ea * (1.0 - c));
+    const double wc = 1.0 + ea * (1.0 - c);  // E0 / E'
+    const double es = e0 / wc;
     const double res_s = R.res * sqrt(es / e0);
     const double k2 = 2.0 * res_s / (double)2.35482f;
     if ((es * (1.0 + k2)) <= R.lo || (es * (1.0 - k2)) >= R.hi) return 0.0;
@@ -493,21 +495,33 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     }
     // cos / sin of theta_min directly, theta_max - theta_min from one atan2
     const double smax = sqrt(fmax(0.0, 1.0 - cmin * cmin));  // sin theta_max
-    double ua, sa, dth;
+    double ua, sa, dy, dx;  // theta_max - theta_min = atan2(dy, dx)
     if (through) {
         ua = 1.0;
         sa = 0.0;
-        dth = atan2(smax, cmin);
+        dy = smax;
+        dx = cmin;
     } else {
         ua = cmax;
         sa = sqrt(fmax(0.0, 1.0 - cmax * cmax));
-        dth = atan2(smax * cmax - cmin * sa, cmin * cmax + smax * sa);
+        dy = smax * cmax - cmin * sa;
+        dx = cmin * cmax + smax * sa;
     }
-    const int nstep = min(kKnSteps, max(0, (int)(dth / p.h)));
+    // step count floor((theta_max - theta_min) / h): FP32 atan2 (error below 1e-6 rad,
+    // 1e-4 of a step) decides it unless the quotient lies within 5e-4 of a step boundary,
+    // where the FP64 atan2 does -- the same count as FP64 throughout
+    const float nf = atan2f((float)dy, (float)dx) * p.ihf;
+    const float fr = nf - floorf(nf);
+    int nraw;
+    if (fr > 5e-4f && fr < 1.0f - 5e-4f)
+        nraw = (int)nf;
+    else
+        nraw = (int)(atan2(dy, dx) / p.h);
+    const int nstep = min(kKnSteps, max(0, nraw));
     const double2 cs = kn[nstep];
-    const double ratio = kn_sum_cs(kc, ua, sa, cs.x, cs.y, p.h) / p.kn_total;
+    const double ratio = kn_sum_cs(kc, ua, sa, cs.x, cs.y, p.h) * p.ikn_total;
     // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
-    const double x = e0 / es, x3 = x * x * x;
+    const double x = wc, x3 = x * x * x;
     const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
     const double pe_abs = R.mu_pe * x3, tot_abs = pe_abs + R.mu_c * x;
     const float absorb = -expm1f(-(float)((double)S.l_abs * tot_abs));
