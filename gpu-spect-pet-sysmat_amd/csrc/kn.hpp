@@ -58,7 +58,8 @@ KN_HD inline void kn_fP(const KnCoef& c, double u, double s, double P, double& f
 KN_HD inline double kn_sum_cs(const KnCoef& c, double ua, double sa, double cnh, double snh, double h) {
     const double ub = ua * cnh - sa * snh, sb = sa * cnh + ua * snh;
     const double wa = 1.0 + c.alpha * (1.0 - ua), wb = 1.0 + c.alpha * (1.0 - ub);
-    const double iwa = 1.0 / wa, iwb = 1.0 / wb;
+    const double iab = 1.0 / (wa * wb);  // one division for both reciprocals
+    const double iwa = wb * iab, iwb = wa * iab;
     double fa, dfa, fb, dfb;
     kn_fP(c, ua, sa, iwa, fa, dfa);
     kn_fP(c, ub, sb, iwb, fb, dfb);

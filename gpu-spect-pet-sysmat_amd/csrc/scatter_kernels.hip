@@ -393,7 +393,8 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
 // ---------------------------------------------------------------------------
 __device__ inline float window_probability(double e, double res, double lo, double hi) {
     // 0.5 (erf(z2) - erf(z1)), z = (bound - E') / (sqrt(2) sigma); tails via erfc
-    const double s2 = res / (double)2.35482f * e * 1.4142135623730951;
+    constexpr double kFwhm = (double)2.35482f;  // FWHM / sigma (reference constant)
+    const double s2 = res * (1.0 / kFwhm) * e * 1.4142135623730951;
     const double is2 = 1.0 / s2;  // one division for both bounds
     const float z1 = (float)((lo - e) * is2), z2 = (float)((hi - e) * is2);
     if (z1 >= 0.0f) return 0.5f * (erfcf(z1) - erfcf(z2));
@@ -456,14 +457,19 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     const double two_pi = 2.0 * M_PI;
     // cone angle at the site (scatter.cu:996-1015, 1246-1248)
     const double ax = S.sx - q.xI, ay = S.sy - q.yI, az = S.sz - q.zI;
-    const double ma = sqrt(ax * ax + ay * ay + az * az);
-    const double ima = 1.0 / ma;
+    // 1 / |S - I| from the hardware reciprocal square root and two Newton steps
+    // (relative error ~1e-16, as sqrt followed by a division)
+    const double ma2 = ax * ax + ay * ay + az * az;
+    double ima = __builtin_amdgcn_rsq(ma2);
+    ima = ima * fma(-0.5 * ma2 * ima, ima, 1.5);
+    ima = ima * fma(-0.5 * ma2 * ima, ima, 1.5);
     double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
     c = fmin(1.0, fmax(-1.0, c));
     const double wc = 1.0 + ea * (1.0 - c);  // E0 / E'
     const double es = e0 / wc;
-    const double res_s = R.res * sqrt(es / e0);
-    const double k2 = 2.0 * res_s / (double)2.35482f;
+    // divisions by loop-invariant quantities as products (the reciprocals are hoisted)
+    const double res_s = R.res * sqrt(es * (1.0 / e0));
+    const double k2 = res_s * (2.0 / (double)2.35482f);
     if ((es * (1.0 + k2)) <= R.lo || (es * (1.0 - k2)) >= R.hi) return 0.0;
     const float p_e = window_probability(es, res_s, R.lo, R.hi);
     const double sn = sqrt(fmax(0.0, 1.0 - c * c));
@@ -471,7 +477,7 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     if (S.L * sn * 2.0 <= R.rdet)
         phi = (float)two_pi;
     else
-        phi = 4.0f * asinf(fminf((float)(R.rdet / 2.0 / S.L / sn), 1.0f));
+        phi = 4.0f * asinf(fminf((float)(R.rdet * 0.5 / (S.L * sn)), 1.0f));
     // theta range over the corners of r (scatter.cu:1303-1336)
     const double asx = S.ssx - q.xIs, asy = S.ssy - q.yIs, asz = S.ssz - q.zIs;
     const double d0 = asx * S.ssx + asy * S.ssy + asz * S.ssz;
