@@ -451,6 +451,7 @@ __device__ inline double site_weight(const ScParams& p, const ScRow& R, const Sc
 // Contribution of one site to element (r, voxel) with source weight w
 // (scatter.cu:1240-1427 / 1736-1923).
 // kn: the (cos n h, sin n h) table (a workgroup's LDS copy in the scatter kernel).
+template <bool kSeries>
 __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPair& S, const VoxelPt& q,
                                    const KnCoef& kc, double w, const double2* __restrict__ kn) {
     const double e0 = p.e0, ea = p.e0 / 511.0;
@@ -525,7 +526,7 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
         nraw = (int)(atan2(dy, dx) / p.h);
     const int nstep = min(kKnSteps, max(0, nraw));
     const double2 cs = kn[nstep];
-    const double ratio = kn_sum_cs(kc, ua, sa, cs.x, cs.y, p.h) * p.ikn_total;
+    const double ratio = kn_sum_cs<kSeries>(kc, ua, sa, cs.x, cs.y, p.h) * p.ikn_total;
     // attenuation with mu scaled to E' (pe ~ E^-3, Compton ~ E^-1)
     const double x = wc, x3 = x * x * x;
     const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
@@ -540,6 +541,9 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
 // against 2 (176 VGPRs, no spill): -29 % on the C3 crystal scatter, -37 % on rows
 // of the 16384-crystal detector; 3 waves were in between, 5-8 waves spill more
 // and lose again.
+// kSeries: the KN logarithm by its series (photon energies up to 236 keV; launch_scatter
+// picks the instance), which keeps the library log1p and its registers out of the kernel.
+template <bool kSeries>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) scatter_kernel(ScParams p, const ScRow* __restrict__ rows,
                                                       const ScPair* __restrict__ pairs,
                                                       const int* __restrict__ counts, const float* __restrict__ pe,
@@ -651,7 +655,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
                 }
                 if (!active) continue;
                 const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
-                acc += site_term(p, R, S, q, kc, w, s_kn);
+                acc += site_term<kSeries>(p, R, S, q, kc, w, s_kn);
             }
         }
     }
@@ -705,7 +709,10 @@ __global__ void __launch_bounds__(256) surrogate_rank_kernel(ScParams p, Surroga
         const int s = k.cand[(long long)r * k.n_cand + t];
         if (s >= 0 && s != r && make_pair(p, R, r, s, cry, nullptr, bits, B, P)) {
             const VoxelPt q = voxel_point(p, R, p.sx, p.sy, p.sz);
-            const double g = site_term(p, R, P, q, kn_coef(p.e0), (double)(float)(R.mu_c / R.mu_pe), p.kn_tab);
+            const double wpe = (double)(float)(R.mu_c / R.mu_pe);
+            const double g = (p.e0 / 511.0 <= kKnSeriesMaxAlpha)
+                                 ? site_term<true>(p, R, P, q, kn_coef(p.e0), wpe, p.kn_tab)
+                                 : site_term<false>(p, R, P, q, kn_coef(p.e0), wpe, p.kn_tab);
             key = g > 0.0 ? (float)g : 0.0f;
             P.pad = key;
             // weight without its angular factor at the image centre (voxel-dependent ring prior)
@@ -795,12 +802,13 @@ hipError_t launch_scatter_prep(const ScParams& p, const ScRow* rows, const Cryst
 hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pairs, const int* counts,
                           const float* pe, float* dst, hipStream_t stream) {
     const long long max_blocks = (1LL << 32) / 256 - 8;  // gridDim.x * blockDim.x < 2^32
+    const auto scatter_kernel_e = (p.e0 / 511.0 <= kKnSeriesMaxAlpha) ? scatter_kernel<true> : scatter_kernel<false>;
     if (p.items) {
         for (long long i0 = 0; i0 < p.n_items; i0 += max_blocks * 4) {
             ScParams q = p;
             q.items = p.items + i0;
             q.n_items = std::min(p.n_items - i0, max_blocks * 4);
-            hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)((q.n_items + 3) / 4)), dim3(256), 0, stream, q, rows,
+            hipLaunchKernelGGL(scatter_kernel_e, dim3((unsigned)((q.n_items + 3) / 4)), dim3(256), 0, stream, q, rows,
                                pairs, counts, pe, dst);
             const hipError_t e = hipGetLastError();
             if (e != hipSuccess) return e;
@@ -823,7 +831,7 @@ hipError_t launch_scatter(const ScParams& p, const ScRow* rows, const ScPair* pa
         q.n_blocks = (long long)q.n_tiles * q.n_rgroups;
         const bool shift_dst = !(p.row_list && p.dst_by_row);
         const long long grid = (q.n_blocks + 7) / 8 * 8;
-        hipLaunchKernelGGL(scatter_kernel, dim3((unsigned)grid), dim3(256), 0, stream, q, rows,
+        hipLaunchKernelGGL(scatter_kernel_e, dim3((unsigned)grid), dim3(256), 0, stream, q, rows,
                            pairs + (long long)d0 * p.max_sites, counts + (long long)d0 * site_chunks(p), pe,
                            shift_dst ? dst + (long long)d0 * p.row_stride : dst);
         const hipError_t e = hipGetLastError();

@@ -19,8 +19,14 @@ SRC = r'''
 #include "kn.hpp"
 extern "C" double kn_cs(double e0, double a, int n, double h) {
     const sysmat::KnCoef c = sysmat::kn_coef(e0);
-    return sysmat::kn_sum_cs(c, std::cos(a), std::sin(a), std::cos((double)n * h), std::sin((double)n * h), h);
+    return sysmat::kn_sum_cs<false>(c, std::cos(a), std::sin(a), std::cos((double)n * h), std::sin((double)n * h), h);
 }
+extern "C" double kn_cs_series(double e0, double a, int n, double h) {
+    const sysmat::KnCoef c = sysmat::kn_coef(e0);
+    return sysmat::kn_sum_cs<true>(c, std::cos(a), std::sin(a), std::cos((double)n * h), std::sin((double)n * h), h);
+}
+extern "C" double kn_log1p(double x) { return sysmat::log1p_series(x); }
+extern "C" double kn_series_max_alpha() { return sysmat::kKnSeriesMaxAlpha; }
 '''
 
 
@@ -35,11 +41,33 @@ def kn(tmp_path_factory):
     L = ctypes.CDLL(str(so))
     L.kn_cs.restype = ctypes.c_double
     L.kn_cs.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.kn_cs_series.restype = ctypes.c_double
+    L.kn_cs_series.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_double]
+    L.kn_log1p.restype = ctypes.c_double
+    L.kn_log1p.argtypes = [ctypes.c_double]
+    L.kn_series_max_alpha.restype = ctypes.c_double
     return L
 
 
-@pytest.mark.parametrize("e0", [140.5, 364.0])
-def test_kn_sum_cs_matches_riemann(kn, e0):
+def test_log1p_series_against_math_log1p(kn):
+    """The series form of ln(1 + x) used for the KN antiderivative difference: within 3 eps of
+    math.log1p wherever the kernel uses it -- the arguments x = w_b / w_a - 1 with w in
+    [1, 1 + 2 alpha] for alpha up to the series bound, the extremes, tiny and zero arguments."""
+    a = kn.kn_series_max_alpha()
+    lo, hi = 1.0 / (1.0 + 2.0 * a) - 1.0, 2.0 * a
+    s_max = a / (1.0 + a)
+    assert s_max * s_max <= 0.1 + 1e-15  # the bound the series is accurate for
+    rng = np.random.default_rng(3)
+    xs = np.concatenate([rng.uniform(lo, hi, 20000), rng.uniform(-1e-3, 1e-3, 2000),
+                         [0.0, 1e-300, -1e-300, 1e-17, lo, hi]])
+    for x in xs:
+        x = float(x)
+        got, want = kn.kn_log1p(x), math.log1p(x)
+        assert abs(got - want) <= 3 * 2.220446049250313e-16 * abs(want) + 1e-320, (x, got, want)
+
+
+@pytest.mark.parametrize("e0,series", [(140.5, False), (364.0, False), (140.5, True), (236.0, True)])
+def test_kn_sum_cs_matches_riemann(kn, e0, series):
     L = oracle.lib()
     h = float(np.float32(0.01))
     rng = np.random.default_rng(0)
@@ -48,7 +76,7 @@ def test_kn_sum_cs_matches_riemann(kn, e0):
         a = float(rng.uniform(0.0, math.pi))
         n = int(rng.integers(0, int((math.pi - a) / h) + 1))
         ref = L.or_kn_riemann_f64(e0, a, a + (n + 0.5) * h)   # i = 0 .. floor(n + 0.5) = n: n + 1 terms
-        got = kn.kn_cs(e0, a, n, h)
+        got = (kn.kn_cs_series if series else kn.kn_cs)(e0, a, n, h)
         if ref > 1e-12:
             worst = max(worst, abs(got - ref) / ref)
     assert worst < 1e-8, worst

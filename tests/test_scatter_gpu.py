@@ -101,6 +101,23 @@ def test_rotation_and_window_variants(lib):
                   oracle.scatter_collimator(g, hb, precision="f64", rot=1), what="collimator rot")
 
 
+@pytest.mark.parametrize("energy", [140.5, 364.0])
+def test_photon_energy_both_kn_logarithms(lib, energy):
+    """The scatter kernel takes the KN logarithm by its series up to 236 keV and from the
+    library above (two kernel instances): 140.5 keV (Tc-99m) and 364 keV (I-131), per-crystal
+    windows (1 +- res/2) E0, both scatter kinds against the FP64 oracle."""
+    g = geo_small(6, side=3, layers=2, image=(3, 3, 4))
+    g.phy[7] = energy
+    g.phy[4] = 0
+    pe = oracle.pe(g, precision="f64", sub=(2, 2, 2))
+    cb, hb = oracle.bitmap_crystal(g), oracle.bitmap_collimator(g)
+    want = oracle.scatter_crystal(g, pe, cb, precision="f64")
+    assert want.max() > 0
+    assert_parity(sc.scatter_crystal(g, pe, cb), want, what=f"crystal {energy} keV")
+    assert_parity(sc.scatter_collimator(g, hb), oracle.scatter_collimator(g, hb, precision="f64"),
+                  what=f"collimator {energy} keV")
+
+
 def test_config3_rows(lib):
     """C3 detector (2 x 32 x 32 crystals, 6x6 holes) on a reduced 8^3 image, GPU bitmaps."""
     g = params.config3()
