@@ -328,6 +328,29 @@ __device__ inline bool bundle_take_hole(const Hole& H, const float4* box) {
     return !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
 }
 
+// Tile group grp: tiles [tx0, tx1) x [ty0, ty1) x [tz0, tz1), voxels [vx0, vx1) x ...
+struct GroupBox {
+    int tx0, ty0, tz0, tx1, ty1, tz1;
+    int vx0, vy0, vz0, vx1, vy1, vz1;
+};
+
+__device__ inline GroupBox group_box(const PeParams& p, long long grp) {
+    GroupBox G;
+    G.tx0 = (int)(grp % p.ngx) * p.gx;
+    G.ty0 = (int)((grp / p.ngx) % p.ngy) * p.gy;
+    G.tz0 = (int)(grp / ((long long)p.ngx * p.ngy)) * p.gz;
+    G.tx1 = min(G.tx0 + p.gx, p.ntx);
+    G.ty1 = min(G.ty0 + p.gy, p.nty);
+    G.tz1 = min(G.tz0 + p.gz, p.ntz);
+    G.vx0 = G.tx0 * p.tx;
+    G.vy0 = G.ty0 * p.ty;
+    G.vz0 = G.tz0 * p.tz;
+    G.vx1 = min(G.tx1 * p.tx, p.X);
+    G.vy1 = min(G.ty1 * p.ty, p.Y);
+    G.vz1 = min(G.tz1 * p.tz, p.Zslab);
+    return G;
+}
+
 // Half extents of the sub-point clouds: voxel sub-points around a voxel centre
 // (rotated frame) and the sub-cell centres of crystal D around its centre (world
 // AABB of the rotated cloud).
@@ -461,26 +484,26 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
     const int wave = tid >> 6;
     const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
 
+    // workgroup = one detector row x tile groups [grp_begin, grp_end): the detector's
+    // set-up below (sub-cell table, bundle end boxes, plate sums) and the register
+    // spills of that set-up are paid once for all of them
     const long long wg = blockIdx.x;
     const int drow = (int)(wg % p.det_count);
-    const long long grp = wg / p.det_count;
+    const long long grp_begin = (wg / p.det_count) * p.grp_per_wg;
+    const long long grp_end = min(grp_begin + p.grp_per_wg, (long long)p.ngx * p.ngy * p.ngz);
     const int d = p.det_begin + drow;
-    // tile group [tx0, tx1) x [ty0, ty1) x [tz0, tz1) of this workgroup
-    const int tx0 = (int)(grp % p.ngx) * p.gx;
-    const int ty0 = (int)((grp / p.ngx) % p.ngy) * p.gy;
-    const int tz0 = (int)(grp / ((long long)p.ngx * p.ngy)) * p.gz;
-    const int tx1 = min(tx0 + p.gx, p.ntx), ty1 = min(ty0 + p.gy, p.nty), tz1 = min(tz0 + p.gz, p.ntz);
-    const int gvx0 = tx0 * p.tx, gvy0 = ty0 * p.ty, gvz0 = tz0 * p.tz;
-    const int gvx1 = min(tx1 * p.tx, p.X), gvy1 = min(ty1 * p.ty, p.Y), gvz1 = min(tz1 * p.tz, p.Zslab);
 
     const Crystal D = cry[d];
     float* row = dst + (long long)drow * p.row_stride;
 
     if (!(D.mu_t > 0.01f)) {  // no culling, no sub-rays: the element is 0 (PESysMatGen.cu:504, 598)
-        const int nvx = gvx1 - gvx0, nvy = gvy1 - gvy0, nvox = nvx * nvy * (gvz1 - gvz0);
-        for (int e = tid; e < nvox; e += kThreads) {
-            const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
-            row[(long long)(gvz0 + lz) * p.X * p.Y + (gvy0 + ly) * p.X + (gvx0 + lx)] = 0.0f;
+        for (long long grp = grp_begin; grp < grp_end; ++grp) {
+            const GroupBox G = group_box(p, grp);
+            const int nvx = G.vx1 - G.vx0, nvy = G.vy1 - G.vy0, nvox = nvx * nvy * (G.vz1 - G.vz0);
+            for (int e = tid; e < nvox; e += kThreads) {
+                const int lx = e % nvx, ly = (e / nvx) % nvy, lz = e / (nvx * nvy);
+                row[(long long)(G.vz0 + lz) * p.X * p.Y + (G.vy0 + ly) * p.X + (G.vx0 + lx)] = 0.0f;
+            }
         }
         return;
     }
@@ -546,495 +569,503 @@ pe_tile_kernel(PeParams p, const Crystal* __restrict__ cry, const Hole* __restri
         }
     }
 
-    // ---- group cull: the group's voxel-centre sphere over the detector's
-    //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
-    //      then culls these shorter lists (a group of one tile writes its tile
-    //      lists directly).  Ordered compaction keeps every list sorted by index.
-    const bool grouped = (tx1 - tx0) * (ty1 - ty0) * (tz1 - tz0) > 1;
-    {
-        int ng = 0, nhg = 0;
-        float tcx, tcy, tcz, rho;
-        box_sphere(p, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, tcx, tcy, tcz, rho);
-        const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
-        const bool gb_ok = group_bundle(p, slb, D, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, pex, pey, hvz, qex, qhy, qez,
-                                        lane, wave, s_tbox);
-        const long long c0 = L.cry_ptr[d], c1 = L.cry_ptr[d + 1];
-        for (long long base = c0; base < c1; base += kThreads) {
-            const long long i = base + tid;
-            int k = 0;
-            bool take = false;
-            if (i < c1) {
-                k = (int)L.cry_list[i];
-                take = sphere_take_crystal(C, cry, k) && (!gb_ok || bundle_take_crystal(cry, k, s_tbox));
-            }
-            const int slot = block_compact(take, ng, s_wcount, lane, wave, below);
-            if (take && slot < kCapGroup) {
-                if (grouped) s_grp[slot] = (unsigned short)k;
-                else s_cand[slot] = (unsigned short)k;
-            }
-        }
-        const long long h0 = L.hol_ptr[d], h1 = L.hol_ptr[d + 1];
-        for (long long base = h0; base < h1; base += kThreads) {
-            const long long i = base + tid;
-            int h = 0;
-            bool take = false;
-            if (i < h1) {
-                h = (int)L.hol_list[i];
-                const Hole H = hol[h];
-                take = sphere_take_hole(C, H) && (!gb_ok || bundle_take_hole(H, s_tbox));
-            }
-            const int slot = block_compact(take, nhg, s_wcount, lane, wave, below);
-            if (take && slot < kCapHoles) {
-                if (grouped) s_hgrp[slot] = (unsigned short)h;
-                else s_hcand[slot] = (unsigned short)h;
-            }
-        }
-        if (ng > kCapGroup || nhg > kCapHoles) {  // host retries with smaller tiles
-            if (tid == 0) atomicOr(status, 1);
-            return;
-        }
-        if (tid == 0) {
-            s_meta[0] = tx0; s_meta[1] = ty0; s_meta[2] = tz0;
-            s_meta[3] = tx1 - tx0; s_meta[4] = ty1 - ty0; s_meta[5] = tz1 - tz0;
-            s_meta[6] = ng; s_meta[7] = nhg;
-        }
-        __syncthreads();
-    }
-
-    for (int t = 0; t < uniform(s_meta[3] * s_meta[4] * s_meta[5]); ++t) {
-        const int gnx = uniform(s_meta[3]), gny = uniform(s_meta[4]);
-        const int tix = uniform(s_meta[0]) + t % gnx, tiy = uniform(s_meta[1]) + (t / gnx) % gny;
-        const int tiz = uniform(s_meta[2]) + t / (gnx * gny);
-        const int vx0 = tix * p.tx, vy0 = tiy * p.ty, vz0 = tiz * p.tz;
-        const int vx1 = min(vx0 + p.tx, p.X), vy1 = min(vy0 + p.ty, p.Y), vz1 = min(vz0 + p.tz, p.Zslab);
-        const int nvx = vx1 - vx0, nvy = vy1 - vy0;
-        const int nvox = nvx * nvy * (vz1 - vz0);
-
-        // ---- tile cull: superset of every element's list ---------------------
-        int nc = uniform(s_meta[6]), nh = uniform(s_meta[7]);
-        if (grouped) {
-            nc = 0;
-            nh = 0;
+    for (long long grp = grp_begin; grp < grp_end; ++grp) {
+        // ---- group cull: the group's voxel-centre sphere over the detector's
+        //      slab-level candidates (pe_detector_cull_kernel) into LDS; each tile
+        //      then culls these shorter lists (a group of one tile writes its tile
+        //      lists directly).  Ordered compaction keeps every list sorted by index.
+        //      The previous group's last barrier orders these LDS writes after its
+        //      reads, and group_bundle's barrier orders the s_meta write below after
+        //      every wave's last read of the previous group's s_meta.
+        const GroupBox G = group_box(p, grp);
+        const int tx0 = G.tx0, ty0 = G.ty0, tz0 = G.tz0, tx1 = G.tx1, ty1 = G.ty1, tz1 = G.tz1;
+        const int gvx0 = G.vx0, gvy0 = G.vy0, gvz0 = G.vz0, gvx1 = G.vx1, gvy1 = G.vy1, gvz1 = G.vz1;
+        const bool grouped = (tx1 - tx0) * (ty1 - ty0) * (tz1 - tz0) > 1;
+        {
+            int ng = 0, nhg = 0;
             float tcx, tcy, tcz, rho;
-            box_sphere(p, vx0, vy0, vz0, vx1, vy1, vz1, tcx, tcy, tcz, rho);
+            box_sphere(p, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, tcx, tcy, tcz, rho);
             const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
-            const bool tb_ok = group_bundle(p, slb, D, vx0, vy0, vz0, vx1, vy1, vz1, pex, pey, hvz, qex, qhy, qez,
+            const bool gb_ok = group_bundle(p, slb, D, gvx0, gvy0, gvz0, gvx1, gvy1, gvz1, pex, pey, hvz, qex, qhy, qez,
                                             lane, wave, s_tbox);
-            const int ng = uniform(s_meta[6]), nhg = uniform(s_meta[7]);
-            for (int base = 0; base < ng; base += kThreads) {
-                const int i = base + tid;
+            const long long c0 = L.cry_ptr[d], c1 = L.cry_ptr[d + 1];
+            for (long long base = c0; base < c1; base += kThreads) {
+                const long long i = base + tid;
                 int k = 0;
                 bool take = false;
-                if (i < ng) {
-                    k = (int)s_grp[i];
-                    take = sphere_take_crystal(C, cry, k) && (!tb_ok || bundle_take_crystal(cry, k, s_tbox));
+                if (i < c1) {
+                    k = (int)L.cry_list[i];
+                    take = sphere_take_crystal(C, cry, k) && (!gb_ok || bundle_take_crystal(cry, k, s_tbox));
                 }
-                const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
-                if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
+                const int slot = block_compact(take, ng, s_wcount, lane, wave, below);
+                if (take && slot < kCapGroup) {
+                    if (grouped) s_grp[slot] = (unsigned short)k;
+                    else s_cand[slot] = (unsigned short)k;
+                }
             }
-            for (int base = 0; base < nhg; base += kThreads) {
-                const int i = base + tid;
+            const long long h0 = L.hol_ptr[d], h1 = L.hol_ptr[d + 1];
+            for (long long base = h0; base < h1; base += kThreads) {
+                const long long i = base + tid;
                 int h = 0;
                 bool take = false;
-                if (i < nhg) {
-                    h = (int)s_hgrp[i];
+                if (i < h1) {
+                    h = (int)L.hol_list[i];
                     const Hole H = hol[h];
-                    take = sphere_take_hole(C, H) && (!tb_ok || bundle_take_hole(H, s_tbox));
+                    take = sphere_take_hole(C, H) && (!gb_ok || bundle_take_hole(H, s_tbox));
                 }
-                const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
-                if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
+                const int slot = block_compact(take, nhg, s_wcount, lane, wave, below);
+                if (take && slot < kCapHoles) {
+                    if (grouped) s_hgrp[slot] = (unsigned short)h;
+                    else s_hcand[slot] = (unsigned short)h;
+                }
             }
-        }
-        if (nc > kCapCrystals || nh > kCapHoles) {  // host retries with smaller tiles
-            if (tid == 0) atomicOr(status, 1);
-            return;
-        }
-        // per-element constants of the tile, one thread per element (tiles of up to kElTab
-        // voxels): the voxel-centre terms of element_consts once per element instead of on
-        // every lane of its wave; larger tiles evaluate them in the element loop
-        const bool el_tab = nvox <= kElTab;
-        if (el_tab && tid < nvox) {
-            ElementConsts c;
-            element_consts(p, D, tid, nvx, nvy, vx0, vy0, vz0, c);
-            s_elA[tid] = make_float4(c.xc0, c.yc0, c.zc0, c.xI);
-            s_elB[tid] = make_float4(c.yI, c.crit_self, c.ux, c.uy);
-            s_elC[tid] = make_float2(c.uz, __int_as_float(c.lx | (c.ly << 10) | (c.lz << 20)));
-        }
-        __syncthreads();  // the candidate lists and element constants are complete before any wave reads them
-        const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
-
-        for (int e = wave; e < nvox; e += kWaves) {
-            ElementConsts ec;
-            if (el_tab) {
-                const float4 a = s_elA[e], b = s_elB[e];
-                const float2 c = s_elC[e];
-                const int pk = __float_as_int(c.y);
-                ec.lx = pk & 0x3FF;
-                ec.ly = (pk >> 10) & 0x3FF;
-                ec.lz = pk >> 20;
-                ec.xc0 = a.x; ec.yc0 = a.y; ec.zc0 = a.z; ec.xI = a.w;
-                ec.yI = b.x; ec.crit_self = b.y; ec.ux = b.z; ec.uy = b.w; ec.uz = c.x;
-            } else {
-                element_consts(p, D, e, nvx, nvy, vx0, vy0, vz0, ec);
+            if (ng > kCapGroup || nhg > kCapHoles) {  // host retries with smaller tiles
+                if (tid == 0) atomicOr(status, 1);
+                return;
             }
-            const int ix = vx0 + ec.lx, iy = vy0 + ec.ly, izl = vz0 + ec.lz;
-            const float xc0 = ec.xc0, yc0 = ec.yc0, zc0 = ec.zc0;
-            const float xI = ec.xI, yI = ec.yI, zI = zc0;
-            const float crit_self = ec.crit_self, ux = ec.ux, uy = ec.uy, uz = ec.uz;
+            if (tid == 0) {
+                s_meta[0] = tx0; s_meta[1] = ty0; s_meta[2] = tz0;
+                s_meta[3] = tx1 - tx0; s_meta[4] = ty1 - ty0; s_meta[5] = tz1 - tz0;
+                s_meta[6] = ng; s_meta[7] = nhg;
+            }
+            __syncthreads();
+        }
 
-            // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
-            const float P_yhi = yI + pey, Q_ylo = D.cy - qhy;
-            const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
-            if (bundle_ok)
-                bundle_boxes(p, slb, lane, xI, yI, zI, pex, pey, hvz, D.cx, D.cy, D.cz, qex, qhy, qez, s_box[wave]);
-            __builtin_amdgcn_wave_barrier();
+        for (int t = 0; t < uniform(s_meta[3] * s_meta[4] * s_meta[5]); ++t) {
+            const int gnx = uniform(s_meta[3]), gny = uniform(s_meta[4]);
+            const int tix = uniform(s_meta[0]) + t % gnx, tiy = uniform(s_meta[1]) + (t / gnx) % gny;
+            const int tiz = uniform(s_meta[2]) + t / (gnx * gny);
+            const int vx0 = tix * p.tx, vy0 = tiy * p.ty, vz0 = tiz * p.tz;
+            const int vx1 = min(vx0 + p.tx, p.X), vy1 = min(vy0 + p.ty, p.Y), vz1 = min(vz0 + p.tz, p.Zslab);
+            const int nvx = vx1 - vx0, nvy = vy1 - vy0;
+            const int nvox = nvx * nvy * (vz1 - vz0);
 
-            // -- crystals: bundle overlap, then the exact predicate; survivors are
-            //    kept as a bitmask and, when they fit, as compacted boxes
-            int n_att = 0;
-            bool any_rotated = false;  // some listed attenuator needs its own frame
-            for (int w = 0; w < ncw; w += 2) {
-                const int j = w * 32 + lane;
-                bool take = false;
-                Crystal K;
-                int k = 0;
-                if (j < nc) {
-                    k = (int)s_cand[j];
-                    K = cry[k];
-                    take = true;
-                    if (bundle_ok && K.slab >= 0.0f) {
-                        const float4 bb = s_box[wave][(int)K.slab];
-                        const float ex = (K.axis != 0.0f) ? K.hx : K.hxz;
-                        const float ez = (K.axis != 0.0f) ? K.hz : ex;
-                        take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
+            // ---- tile cull: superset of every element's list ---------------------
+            int nc = uniform(s_meta[6]), nh = uniform(s_meta[7]);
+            if (grouped) {
+                nc = 0;
+                nh = 0;
+                float tcx, tcy, tcz, rho;
+                box_sphere(p, vx0, vy0, vz0, vx1, vy1, vz1, tcx, tcy, tcz, rho);
+                const SphereCull C = sphere_cull(p, D, tcx, tcy, tcz, rho);
+                const bool tb_ok = group_bundle(p, slb, D, vx0, vy0, vz0, vx1, vy1, vz1, pex, pey, hvz, qex, qhy, qez,
+                                                lane, wave, s_tbox);
+                const int ng = uniform(s_meta[6]), nhg = uniform(s_meta[7]);
+                for (int base = 0; base < ng; base += kThreads) {
+                    const int i = base + tid;
+                    int k = 0;
+                    bool take = false;
+                    if (i < ng) {
+                        k = (int)s_grp[i];
+                        take = sphere_take_crystal(C, cry, k) && (!tb_ok || bundle_take_crystal(cry, k, s_tbox));
                     }
+                    const int slot = block_compact(take, nc, s_wcount, lane, wave, below);
+                    if (take && slot < kCapCrystals) s_cand[slot] = (unsigned short)k;
+                }
+                for (int base = 0; base < nhg; base += kThreads) {
+                    const int i = base + tid;
+                    int h = 0;
+                    bool take = false;
+                    if (i < nhg) {
+                        h = (int)s_hgrp[i];
+                        const Hole H = hol[h];
+                        take = sphere_take_hole(C, H) && (!tb_ok || bundle_take_hole(H, s_tbox));
+                    }
+                    const int slot = block_compact(take, nh, s_wcount, lane, wave, below);
+                    if (take && slot < kCapHoles) s_hcand[slot] = (unsigned short)h;
+                }
+            }
+            if (nc > kCapCrystals || nh > kCapHoles) {  // host retries with smaller tiles
+                if (tid == 0) atomicOr(status, 1);
+                return;
+            }
+            // per-element constants of the tile, one thread per element (tiles of up to kElTab
+            // voxels): the voxel-centre terms of element_consts once per element instead of on
+            // every lane of its wave; larger tiles evaluate them in the element loop
+            const bool el_tab = nvox <= kElTab;
+            if (el_tab && tid < nvox) {
+                ElementConsts c;
+                element_consts(p, D, tid, nvx, nvy, vx0, vy0, vz0, c);
+                s_elA[tid] = make_float4(c.xc0, c.yc0, c.zc0, c.xI);
+                s_elB[tid] = make_float4(c.yI, c.crit_self, c.ux, c.uy);
+                s_elC[tid] = make_float2(c.uz, __int_as_float(c.lx | (c.ly << 10) | (c.lz << 20)));
+            }
+            __syncthreads();  // the candidate lists and element constants are complete before any wave reads them
+            const int ncw = (nc + 31) >> 5, nhw = (nh + 31) >> 5;
+
+            for (int e = wave; e < nvox; e += kWaves) {
+                ElementConsts ec;
+                if (el_tab) {
+                    const float4 a = s_elA[e], b = s_elB[e];
+                    const float2 c = s_elC[e];
+                    const int pk = __float_as_int(c.y);
+                    ec.lx = pk & 0x3FF;
+                    ec.ly = (pk >> 10) & 0x3FF;
+                    ec.lz = pk >> 20;
+                    ec.xc0 = a.x; ec.yc0 = a.y; ec.zc0 = a.z; ec.xI = a.w;
+                    ec.yI = b.x; ec.crit_self = b.y; ec.ux = b.z; ec.uy = b.w; ec.uz = c.x;
+                } else {
+                    element_consts(p, D, e, nvx, nvy, vx0, vy0, vz0, ec);
+                }
+                const int ix = vx0 + ec.lx, iy = vy0 + ec.ly, izl = vz0 + ec.lz;
+                const float xc0 = ec.xc0, yc0 = ec.yc0, zc0 = ec.zc0;
+                const float xI = ec.xI, yI = ec.yI, zI = zc0;
+                const float crit_self = ec.crit_self, ux = ec.ux, uy = ec.uy, uz = ec.uz;
+
+                // -- bundle bounds per y-slab: P box (voxel sub-points), Q box (sub-cell centres of d)
+                const float P_yhi = yI + pey, Q_ylo = D.cy - qhy;
+                const bool bundle_ok = p.use_bundle && (Q_ylo - P_yhi > 1.0f);
+                if (bundle_ok)
+                    bundle_boxes(p, slb, lane, xI, yI, zI, pex, pey, hvz, D.cx, D.cy, D.cz, qex, qhy, qez, s_box[wave]);
+                __builtin_amdgcn_wave_barrier();
+
+                // -- crystals: bundle overlap, then the exact predicate; survivors are
+                //    kept as a bitmask and, when they fit, as compacted boxes
+                int n_att = 0;
+                bool any_rotated = false;  // some listed attenuator needs its own frame
+                for (int w = 0; w < ncw; w += 2) {
+                    const int j = w * 32 + lane;
+                    bool take = false;
+                    Crystal K;
+                    int k = 0;
+                    if (j < nc) {
+                        k = (int)s_cand[j];
+                        K = cry[k];
+                        take = true;
+                        if (bundle_ok && K.slab >= 0.0f) {
+                            const float4 bb = s_box[wave][(int)K.slab];
+                            const float ex = (K.axis != 0.0f) ? K.hx : K.hxz;
+                            const float ez = (K.axis != 0.0f) ? K.hz : ex;
+                            take = !(K.cx + ex < bb.x || K.cx - ex > bb.y || K.cz + ez < bb.z || K.cz - ez > bb.w);
+                        }
+                        if (take) {
+                            const int fast =
+                                cull_fast(K.cx - xI, K.cy - yI, K.cz - zI, 0.5f * K.rbound, ux, uy, uz, crit_self);
+                            if (fast != 0) {
+                                take = fast > 0;
+                            } else {  // near the boundary: the reference's operations
+                                float crit = K.rbound;
+                                const float dk =
+                                    sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) +
+                                          (zI - K.cz) * (zI - K.cz));
+                                crit = crit / dk / 2.0f;
+                                const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
+                                take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                                       crit + crit_self;
+                            }
+                        }
+                    }
+                    const unsigned long long bal = __ballot(take);
+                    if (__ballot(take && K.axis == 0.0f)) any_rotated = true;
                     if (take) {
-                        const int fast =
-                            cull_fast(K.cx - xI, K.cy - yI, K.cz - zI, 0.5f * K.rbound, ux, uy, uz, crit_self);
-                        if (fast != 0) {
-                            take = fast > 0;
-                        } else {  // near the boundary: the reference's operations
-                            float crit = K.rbound;
-                            const float dk =
-                                sqrtf((xI - K.cx) * (xI - K.cx) + (yI - K.cy) * (yI - K.cy) +
-                                      (zI - K.cz) * (zI - K.cz));
-                            crit = crit / dk / 2.0f;
-                            const float vx = (K.cx - xI) / dk, vy = (K.cy - yI) / dk, vz = (K.cz - zI) / dk;
-                            take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                                   crit + crit_self;
+                        const int slot = n_att + __popcll(bal & below);
+                        if (slot < kListCap) {
+                            Att A;
+                            A.x1 = K.cx - K.hx; A.x2 = K.cx + K.hx;
+                            A.y1 = K.cy - K.hy; A.y2 = K.cy + K.hy;
+                            A.z1 = K.cz - K.hz; A.z2 = K.cz + K.hz;
+                            A.mu = K.mu_t;
+                            A.k = (K.axis != 0.0f) ? k : -(k + 1);
+                            s_att[wave][slot] = A;
                         }
                     }
-                }
-                const unsigned long long bal = __ballot(take);
-                if (__ballot(take && K.axis == 0.0f)) any_rotated = true;
-                if (take) {
-                    const int slot = n_att + __popcll(bal & below);
-                    if (slot < kListCap) {
-                        Att A;
-                        A.x1 = K.cx - K.hx; A.x2 = K.cx + K.hx;
-                        A.y1 = K.cy - K.hy; A.y2 = K.cy + K.hy;
-                        A.z1 = K.cz - K.hz; A.z2 = K.cz + K.hz;
-                        A.mu = K.mu_t;
-                        A.k = (K.axis != 0.0f) ? k : -(k + 1);
-                        s_att[wave][slot] = A;
+                    n_att += __popcll(bal);
+                    if (lane == 0) {
+                        s_mask[wave][w] = (unsigned)bal;
+                        if (w + 1 < (kCapCrystals / 32)) s_mask[wave][w + 1] = (unsigned)(bal >> 32);
                     }
                 }
-                n_att += __popcll(bal);
-                if (lane == 0) {
-                    s_mask[wave][w] = (unsigned)bal;
-                    if (w + 1 < (kCapCrystals / 32)) s_mask[wave][w + 1] = (unsigned)(bal >> 32);
-                }
-            }
-            // -- holes: bundle overlap, exact predicate; layer_holes bit m set when
-            //    some hole of layer m survives
-            unsigned layer_holes = 0;
-            int n_hol = 0;
-            for (int w = 0; w < nhw; w += 2) {
-                const int j = w * 32 + lane;
-                bool take = false;
-                int h = 0;
-                if (j < nh) {
-                    h = (int)s_hcand[j];
-                    const Hole H = hol[h];
-                    take = true;
-                    if (bundle_ok && H.slab >= 0.0f) {
-                        const float4 bb = s_box[wave][(int)H.slab];
-                        take = !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+                // -- holes: bundle overlap, exact predicate; layer_holes bit m set when
+                //    some hole of layer m survives
+                unsigned layer_holes = 0;
+                int n_hol = 0;
+                for (int w = 0; w < nhw; w += 2) {
+                    const int j = w * 32 + lane;
+                    bool take = false;
+                    int h = 0;
+                    if (j < nh) {
+                        h = (int)s_hcand[j];
+                        const Hole H = hol[h];
+                        take = true;
+                        if (bundle_ok && H.slab >= 0.0f) {
+                            const float4 bb = s_box[wave][(int)H.slab];
+                            take = !(H.x + H.r < bb.x || H.x - H.r > bb.y || H.z + H.r < bb.z || H.z - H.r > bb.w);
+                        }
+                        if (take) {
+                            const float yh = (H.y1 + H.y2) / 2.0f;
+                            const int fast = cull_fast(H.x - xI, yh - yI, H.z - zI, H.rbound, ux, uy, uz, crit_self);
+                            if (fast != 0) {
+                                take = fast > 0;
+                            } else {  // near the boundary: the reference's operations
+                                float crit = H.rbound;
+                                const float dh =
+                                    sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
+                                crit = crit / dh;
+                                const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
+                                take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
+                                       crit + crit_self;
+                            }
+                        }
+                        if (take) layer_holes |= 1u << (int)H.layer;
                     }
+                    const unsigned long long bal = __ballot(take);
                     if (take) {
-                        const float yh = (H.y1 + H.y2) / 2.0f;
-                        const int fast = cull_fast(H.x - xI, yh - yI, H.z - zI, H.rbound, ux, uy, uz, crit_self);
-                        if (fast != 0) {
-                            take = fast > 0;
-                        } else {  // near the boundary: the reference's operations
-                            float crit = H.rbound;
-                            const float dh =
-                                sqrtf((xI - H.x) * (xI - H.x) + (yI - yh) * (yI - yh) + (zI - H.z) * (zI - H.z));
-                            crit = crit / dh;
-                            const float vx = (H.x - xI) / dh, vy = (yh - yI) / dh, vz = (H.z - zI) / dh;
-                            take = sqrtf((vz - uz) * (vz - uz) + (vy - uy) * (vy - uy) + (vx - ux) * (vx - ux)) <=
-                                   crit + crit_self;
-                        }
+                        const int slot = n_hol + __popcll(bal & below);
+                        if (slot < kHoleCap) s_hol[wave][slot] = h;
                     }
-                    if (take) layer_holes |= 1u << (int)H.layer;
+                    n_hol += __popcll(bal);
+                    if (lane == 0) {
+                        s_hmask[wave][w] = (unsigned)bal;
+                        if (w + 1 < (kCapHoles / 32)) s_hmask[wave][w + 1] = (unsigned)(bal >> 32);
+                    }
                 }
-                const unsigned long long bal = __ballot(take);
-                if (take) {
-                    const int slot = n_hol + __popcll(bal & below);
-                    if (slot < kHoleCap) s_hol[wave][slot] = h;
+                {  // OR over the wave: one ballot per layer
+                    unsigned any = 0;
+                    for (int m = 0; m < p.n_layers; ++m)
+                        if (__ballot((layer_holes >> m) & 1u) != 0ull) any |= 1u << m;
+                    layer_holes = uniform(any);
                 }
-                n_hol += __popcll(bal);
-                if (lane == 0) {
-                    s_hmask[wave][w] = (unsigned)bal;
-                    if (w + 1 < (kCapHoles / 32)) s_hmask[wave][w + 1] = (unsigned)(bal >> 32);
+                n_att = uniform(n_att);
+                n_hol = uniform(n_hol);
+                // -- plate classes: test per sub-ray, every sub-ray passes, every sub-ray misses
+                unsigned plate_test = 0;  // bit m: per-sub-ray test needed
+                bool element_blocked = false;
+                for (int m = 0; m < p.n_layers; ++m) {
+                    const Plate PL = pla[m];
+                    int cls = 0;
+                    if (bundle_ok && PL.slab >= 0.0f) {
+                        const float4 bb = s_box[wave][(int)PL.slab];
+                        const bool crosses = (P_yhi < PL.y1 - 0.01f) && (Q_ylo > PL.y2 + 0.01f) && PL.thick >= 0.2f;
+                        if (crosses && bb.x >= PL.x1 + 0.01f && bb.y <= PL.x2 - 0.01f && bb.z >= PL.z1 + 0.01f &&
+                            bb.w <= PL.z2 - 0.01f)
+                            cls = 1;
+                        else if (bb.y < PL.x1 || bb.x > PL.x2 || bb.w < PL.z1 || bb.z > PL.z2)
+                            cls = 2;
+                    }
+                    if (cls == 0) plate_test |= 1u << m;
+                    if (cls == 2) element_blocked = true;
                 }
-            }
-            {  // OR over the wave: one ballot per layer
-                unsigned any = 0;
-                for (int m = 0; m < p.n_layers; ++m)
-                    if (__ballot((layer_holes >> m) & 1u) != 0ull) any |= 1u << m;
-                layer_holes = uniform(any);
-            }
-            n_att = uniform(n_att);
-            n_hol = uniform(n_hol);
-            // -- plate classes: test per sub-ray, every sub-ray passes, every sub-ray misses
-            unsigned plate_test = 0;  // bit m: per-sub-ray test needed
-            bool element_blocked = false;
-            for (int m = 0; m < p.n_layers; ++m) {
-                const Plate PL = pla[m];
-                int cls = 0;
-                if (bundle_ok && PL.slab >= 0.0f) {
-                    const float4 bb = s_box[wave][(int)PL.slab];
-                    const bool crosses = (P_yhi < PL.y1 - 0.01f) && (Q_ylo > PL.y2 + 0.01f) && PL.thick >= 0.2f;
-                    if (crosses && bb.x >= PL.x1 + 0.01f && bb.y <= PL.x2 - 0.01f && bb.z >= PL.z1 + 0.01f &&
-                        bb.w <= PL.z2 - 0.01f)
-                        cls = 1;
-                    else if (bb.y < PL.x1 || bb.x > PL.x2 || bb.w < PL.z1 || bb.z > PL.z2)
-                        cls = 2;
-                }
-                if (cls == 0) plate_test |= 1u << m;
-                if (cls == 2) element_blocked = true;
-            }
-            __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_wave_barrier();
 
-            double acc = 0.0;
-            if (!element_blocked) {
-                // FP64 voxel centre (before rotation) for the collimator path
-                const double xc64 = ((double)ix - p.hX64 + 0.5) * p.wx64 + p.sx64;
-                const double yc64 = ((double)iy - p.hY64 + 0.5) * p.wy64 + p.sy64;
-                const double zc64 = ((double)(p.z0 + izl) - p.hZ64 + 0.5) * p.wz64 + p.sz64;
-                const bool att_fast = n_att <= kListCap;
-                const bool hol_fast = n_hol <= kHoleCap;
-                for (int base = 0; base < S; base += 64) {
-                    const int s = base + lane;
-                    if (s >= S) break;
-                    // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
-                    int ny, nx, nz, va = 0, vb = 0, vc = 0;
-                    SubCell SC;
-                    if (sub_tab) {
-                        SC = s_sub[s];
-                        ny = (int)(SC.idx0 & 0xFFFFu);
-                        nx = (int)(SC.idx0 >> 16);
-                        nz = (int)(SC.idx1 & 0xFFu);
-                        va = (int)((SC.idx1 >> 8) & 0xFFu);
-                        vb = (int)((SC.idx1 >> 16) & 0xFFu);
-                        vc = (int)(SC.idx1 >> 24);
-                    } else if (idx_tab) {
-                        const uint2 e = s_idx[s];
-                        ny = (int)(e.x & 0xFFFFu);
-                        nx = (int)(e.x >> 16);
-                        nz = (int)(e.y & 0xFFu);
-                        va = (int)((e.y >> 8) & 0xFFu);
-                        vb = (int)((e.y >> 16) & 0xFFu);
-                        vc = (int)(e.y >> 24);
-                        sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
-                    } else {
-                        int q = divmod(s, p.cy, rcy, ny);
-                        q = divmod(q, p.cx, rcx, nx);
-                        const int iv = divmod(q, p.cz, rcz, nz);
-                        if (p.nv > 1) {
-                            const int t1 = divmod(iv, p.nv, rnv, va);
-                            vc = divmod(t1, p.nv, rnv, vb);
+                double acc = 0.0;
+                if (!element_blocked) {
+                    // FP64 voxel centre (before rotation) for the collimator path
+                    const double xc64 = ((double)ix - p.hX64 + 0.5) * p.wx64 + p.sx64;
+                    const double yc64 = ((double)iy - p.hY64 + 0.5) * p.wy64 + p.sy64;
+                    const double zc64 = ((double)(p.z0 + izl) - p.hZ64 + 0.5) * p.wz64 + p.sz64;
+                    const bool att_fast = n_att <= kListCap;
+                    const bool hol_fast = n_hol <= kHoleCap;
+                    for (int base = 0; base < S; base += 64) {
+                        const int s = base + lane;
+                        if (s >= S) break;
+                        // sub-ray index: voxel sub-point outer, then NumZ, NumX, NumY (reference loop order)
+                        int ny, nx, nz, va = 0, vb = 0, vc = 0;
+                        SubCell SC;
+                        if (sub_tab) {
+                            SC = s_sub[s];
+                            ny = (int)(SC.idx0 & 0xFFFFu);
+                            nx = (int)(SC.idx0 >> 16);
+                            nz = (int)(SC.idx1 & 0xFFu);
+                            va = (int)((SC.idx1 >> 8) & 0xFFu);
+                            vb = (int)((SC.idx1 >> 16) & 0xFFu);
+                            vc = (int)(SC.idx1 >> 24);
+                        } else if (idx_tab) {
+                            const uint2 e = s_idx[s];
+                            ny = (int)(e.x & 0xFFFFu);
+                            nx = (int)(e.x >> 16);
+                            nz = (int)(e.y & 0xFFu);
+                            va = (int)((e.y >> 8) & 0xFFu);
+                            vb = (int)((e.y >> 16) & 0xFFu);
+                            vc = (int)(e.y >> 24);
+                            sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
+                        } else {
+                            int q = divmod(s, p.cy, rcy, ny);
+                            q = divmod(q, p.cx, rcx, nx);
+                            const int iv = divmod(q, p.cz, rcz, nz);
+                            if (p.nv > 1) {
+                                const int t1 = divmod(iv, p.nv, rnv, va);
+                                vc = divmod(t1, p.nv, rnv, vb);
+                            }
+                            sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
                         }
-                        sub_cell_geometry(D, nx, ny, nz, wD, tD, hD, rcx, rcy, rcz, cD, sD, SC);
-                    }
-                    float att = 0.0f;
-                    double att64 = 0.0;
-                    // collimator layers, plate length minus hole chords (PESysMatGen.cu:661-685),
-                    // when some hole is a candidate: first, because this FP64 path needs only the
-                    // sub-ray's indices, so its registers are free again before the FP32 ray is built
-                    if (layer_holes != 0) {
-                        // end points in FP64 with the same formulas (quotients by the
-                        // sampling counts through their reciprocals, correctly rounded)
-                        double px = xc64, py = yc64, pz = zc64;
-                        if (p.nv > 1) {
-                            const double fnv = (double)p.nv;
-                            px += (div_rcp((double)va + 0.5, fnv, p.rnv64) - 0.5) * p.wx64;
-                            py += (div_rcp((double)vb + 0.5, fnv, p.rnv64) - 0.5) * p.wy64;
-                            pz += (div_rcp((double)vc + 0.5, fnv, p.rnv64) - 0.5) * p.wz64;
-                        }
-                        const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
-                        const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
-                        const double xs = -w64 / 2.0 + div_rcp((double)nx + 0.5, (double)p.cx, p.rcx64) * w64;
-                        const double zs = -h64 / 2.0 + div_rcp((double)nz + 0.5, (double)p.cz, p.rcz64) * h64;
-                        const double ys = -t64 / 2.0 + div_rcp((double)ny + 0.5, (double)p.cy, p.rcy64) * t64;
-                        const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
-                        const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
-                        const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
-                        const Ray64 R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
-                        for (int m = 0; m < p.n_layers; ++m) {
-                            const Plate PL = pla[m];
-                            double holes_len = 0.0;
-                            if ((layer_holes >> m) & 1u) {
-                                auto one_hole = [&](int h) {
-                                    const Hole H = hol[h];
-                                    if ((int)H.layer != m) return;
-                                    const double c = chord_hole64(R64, H.x, (double)H.y1_raw + (double)p.fov2c,
-                                                                  (double)H.y2_raw + (double)p.fov2c, H.z, H.r);
-                                    holes_len += c;
-                                    att64 += (double)H.mu_t * c;
-                                };
-                                if (hol_fast) {
-                                    for (int j = 0; j < n_hol; ++j) one_hole(uniform(s_hol[wave][j]));
-                                } else {
-                                    for (int w = 0; w < nhw; ++w) {
-                                        unsigned msk = uniform(s_hmask[wave][w]);
-                                        while (msk) {
-                                            const int b = __builtin_ctz(msk);
-                                            msk &= msk - 1;
-                                            one_hole(uniform((int)s_hcand[w * 32 + b]));
+                        float att = 0.0f;
+                        double att64 = 0.0;
+                        // collimator layers, plate length minus hole chords (PESysMatGen.cu:661-685),
+                        // when some hole is a candidate: first, because this FP64 path needs only the
+                        // sub-ray's indices, so its registers are free again before the FP32 ray is built
+                        if (layer_holes != 0) {
+                            // end points in FP64 with the same formulas (quotients by the
+                            // sampling counts through their reciprocals, correctly rounded)
+                            double px = xc64, py = yc64, pz = zc64;
+                            if (p.nv > 1) {
+                                const double fnv = (double)p.nv;
+                                px += (div_rcp((double)va + 0.5, fnv, p.rnv64) - 0.5) * p.wx64;
+                                py += (div_rcp((double)vb + 0.5, fnv, p.rnv64) - 0.5) * p.wy64;
+                                pz += (div_rcp((double)vc + 0.5, fnv, p.rnv64) - 0.5) * p.wz64;
+                            }
+                            const double rpx = px * p.rc64 - py * p.rs64, rpy = px * p.rs64 + py * p.rc64;
+                            const double w64 = 2.0 * (double)D.hx, h64 = 2.0 * (double)D.hz, t64 = 2.0 * (double)D.hy;
+                            const double xs = -w64 / 2.0 + div_rcp((double)nx + 0.5, (double)p.cx, p.rcx64) * w64;
+                            const double zs = -h64 / 2.0 + div_rcp((double)nz + 0.5, (double)p.cz, p.rcz64) * h64;
+                            const double ys = -t64 / 2.0 + div_rcp((double)ny + 0.5, (double)p.cy, p.rcy64) * t64;
+                            const double qx = (double)D.cx + (xs * cD64 - zs * sD64);
+                            const double qz = (double)D.cz + (xs * sD64 + zs * cD64);
+                            const double qy = ((double)D.y_raw + (double)p.fov2c) + ys;
+                            const Ray64 R64 = make_ray64(rpx, rpy, pz, qx, qy, qz);
+                            for (int m = 0; m < p.n_layers; ++m) {
+                                const Plate PL = pla[m];
+                                double holes_len = 0.0;
+                                if ((layer_holes >> m) & 1u) {
+                                    auto one_hole = [&](int h) {
+                                        const Hole H = hol[h];
+                                        if ((int)H.layer != m) return;
+                                        const double c = chord_hole64(R64, H.x, (double)H.y1_raw + (double)p.fov2c,
+                                                                      (double)H.y2_raw + (double)p.fov2c, H.z, H.r);
+                                        holes_len += c;
+                                        att64 += (double)H.mu_t * c;
+                                    };
+                                    if (hol_fast) {
+                                        for (int j = 0; j < n_hol; ++j) one_hole(uniform(s_hol[wave][j]));
+                                    } else {
+                                        for (int w = 0; w < nhw; ++w) {
+                                            unsigned msk = uniform(s_hmask[wave][w]);
+                                            while (msk) {
+                                                const int b = __builtin_ctz(msk);
+                                                msk &= msk - 1;
+                                                one_hole(uniform((int)s_hcand[w * 32 + b]));
+                                            }
                                         }
                                     }
                                 }
+                                const double in_plate = div_rcp((double)PL.thick, R64.ky, R64.iky);
+                                if ((in_plate - holes_len) >= (double)0.00001f)
+                                    att64 += (double)PL.mu_t * (in_plate - holes_len);
                             }
-                            const double in_plate = div_rcp((double)PL.thick, R64.ky, R64.iky);
-                            if ((in_plate - holes_len) >= (double)0.00001f)
-                                att64 += (double)PL.mu_t * (in_plate - holes_len);
                         }
-                    }
-                    float xP = xI, yP = yI, zP = zI;
-                    if (p.nv > 1) {
-                        const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
-                        const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
-                        const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
-                        rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
-                        zP = zc0 + oz;
-                    }
-                    const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
-                    const float yPs = yP - D.cy;
-                    const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
-                    // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
-                    const float xDs = SC.xDs, yDs = SC.yDs, zDs = SC.zDs;
-                    const float xQ = SC.xQ, yQ = SC.yQ, zQ = SC.zQ;
+                        float xP = xI, yP = yI, zP = zI;
+                        if (p.nv > 1) {
+                            const float ox = (((float)va + 0.5f) * rnv - 0.5f) * p.wx;
+                            const float oy = (((float)vb + 0.5f) * rnv - 0.5f) * p.wy;
+                            const float oz = (((float)vc + 0.5f) * rnv - 0.5f) * p.wz;
+                            rotate_image(p, xc0 + ox, yc0 + oy, xP, yP);
+                            zP = zc0 + oz;
+                        }
+                        const float xPs = (xP - D.cx) * cDn - (zP - D.cz) * sDn;
+                        const float yPs = yP - D.cy;
+                        const float zPs = (xP - D.cx) * sDn + (zP - D.cz) * cDn;
+                        // crystal sub-point, crystal frame then world (PESysMatGen.cu:606-616)
+                        const float xDs = SC.xDs, yDs = SC.yDs, zDs = SC.zDs;
+                        const float xQ = SC.xQ, yQ = SC.yQ, zQ = SC.zQ;
 
-                    const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
-                    const float dist = R.t;
-                    // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
-                    const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
-                    const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
-                    const float fy = A_y * fabsf(R.dy);
-                    const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
+                        const WRay R = make_wray(xP, yP, zP, xQ, yQ, zQ);
+                        const float dist = R.t;
+                        // solid angle: largest projected sub-face (PESysMatGen.cu:618-639)
+                        const float fx = A_x * fabsf(R.dx * cD - R.dz * sD);
+                        const float fz = A_z * fabsf(R.dx * sD + R.dz * cD);
+                        const float fy = A_y * fabsf(R.dy);
+                        const float solid = fmaxf(fmaxf(fx, fy), fz) * p.inv4pi * frcp(dist * dist * dist);
 
-                    bool blocked = false;
-                    // plate miss => blocked (PESysMatGen.cu:645-659)
-                    for (unsigned mm = plate_test; mm; mm &= mm - 1) {
-                        const int m = __builtin_ctz(mm);
-                        const Plate PL = pla[m];
-                        const float c =
-                            (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
-                                         : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t, R.pat,
-                                                         PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
-                        if (c < 0.1f) blocked = true;
-                    }
-                    // collimator layers without candidate holes (PESysMatGen.cu:661-685)
-                    if (layer_holes == 0) {
-                        // plate path thick * |PQ| / dy = thick (1 + q), q = r^2 / (1 + sqrt(1 + r^2)),
-                        // r = lateral / dy: mu * thick summed in FP64 and only the small q term
-                        // rounded in FP32, so the exponent of strongly attenuated elements (tens
-                        // of mean free paths) keeps ~1e-7 absolute accuracy.  dy <= 0 gives a
-                        // negative length, which the reference does not add.
-                        if (R.dy > 0.0f) {
-                            const float idy = frcp(R.dy);
-                            const float r2 = (R.dx * R.dx + R.dz * R.dz) * (idy * idy);
-                            const float qv = r2 * frcp(1.0f + fsqrt(1.0f + r2));
-                            att64 += plate_mt + (double)(plate_mtf * qv);
-                            if (thin_plates) {  // plates under 1e-5 mm count only along very oblique paths
-                                for (int m = 0; m < p.n_layers; ++m) {
-                                    const Plate PL = pla[m];
-                                    if (PL.thick < 0.00001f && PL.thick * (1.0f + qv) >= 0.00001f)
-                                        att64 += (double)PL.mu_t * (double)PL.thick +
-                                                 (double)((PL.mu_t * PL.thick) * qv);
+                        bool blocked = false;
+                        // plate miss => blocked (PESysMatGen.cu:645-659)
+                        for (unsigned mm = plate_test; mm; mm &= mm - 1) {
+                            const int m = __builtin_ctz(mm);
+                            const Plate PL = pla[m];
+                            const float c =
+                                (R.pat == 0) ? chord_world(R, PL.x1, PL.x2, PL.y1, PL.y2, PL.z1, PL.z2)
+                                             : chord_special(R.px, R.py, R.pz, xQ, yQ, zQ, R.ix, R.iy, R.iz, R.t, R.pat,
+                                                             PL.x1, PL.y1, PL.z1, PL.x2, PL.y2, PL.z2);
+                            if (c < 0.1f) blocked = true;
+                        }
+                        // collimator layers without candidate holes (PESysMatGen.cu:661-685)
+                        if (layer_holes == 0) {
+                            // plate path thick * |PQ| / dy = thick (1 + q), q = r^2 / (1 + sqrt(1 + r^2)),
+                            // r = lateral / dy: mu * thick summed in FP64 and only the small q term
+                            // rounded in FP32, so the exponent of strongly attenuated elements (tens
+                            // of mean free paths) keeps ~1e-7 absolute accuracy.  dy <= 0 gives a
+                            // negative length, which the reference does not add.
+                            if (R.dy > 0.0f) {
+                                const float idy = frcp(R.dy);
+                                const float r2 = (R.dx * R.dx + R.dz * R.dz) * (idy * idy);
+                                const float qv = r2 * frcp(1.0f + fsqrt(1.0f + r2));
+                                att64 += plate_mt + (double)(plate_mtf * qv);
+                                if (thin_plates) {  // plates under 1e-5 mm count only along very oblique paths
+                                    for (int m = 0; m < p.n_layers; ++m) {
+                                        const Plate PL = pla[m];
+                                        if (PL.thick < 0.00001f && PL.thick * (1.0f + qv) >= 0.00001f)
+                                            att64 += (double)PL.mu_t * (double)PL.thick +
+                                                     (double)((PL.mu_t * PL.thick) * qv);
+                                    }
                                 }
                             }
                         }
-                    }
-                    // other crystals on the way (PESysMatGen.cu:687-725)
-                    if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
-                        // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
-                        // wave; parameters measured from Q (the attenuators lie near the detecting
-                        // crystal); unrolled so the LDS reads of several boxes are in flight together
-#pragma unroll 4
-                        for (int j = 0; j < n_att; ++j) {
-                            const Att A = s_att[wave][j];
-                            att += chord_world_q(R, xQ, yQ, zQ, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
-                        }
-                    } else if (att_fast) {
-                        for (int j = 0; j < n_att; ++j) {
-                            const Att A = s_att[wave][j];
-                            const int k = uniform(A.k);
-                            const bool rotated = k < 0;
-                            att += crystal_chord(R, xQ, yQ, zQ, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2,
-                                                 A.z1, A.z2, rotated) * A.mu;
-                        }
-                    } else {
-                        for (int w = 0; w < ncw; ++w) {
-                            unsigned msk = uniform(s_mask[wave][w]);
-                            while (msk) {
-                                const int b = __builtin_ctz(msk);
-                                msk &= msk - 1;
-                                const int k = uniform((int)s_cand[w * 32 + b]);
-                                const Crystal K = cry[k];
-                                att += crystal_chord(R, xQ, yQ, zQ, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy,
-                                                     K.cy + K.hy, K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) *
-                                       K.mu_t;
+                        // other crystals on the way (PESysMatGen.cu:687-725)
+                        if (att_fast && !any_rotated && __ballot(R.pat != 0) == 0ull) {
+                            // common case, branch-free: axis-aligned boxes, no axis-parallel sub-ray in the
+                            // wave; parameters measured from Q (the attenuators lie near the detecting
+                            // crystal); unrolled so the LDS reads of several boxes are in flight together
+    #pragma unroll 4
+                            for (int j = 0; j < n_att; ++j) {
+                                const Att A = s_att[wave][j];
+                                att += chord_world_q(R, xQ, yQ, zQ, A.x1, A.x2, A.y1, A.y2, A.z1, A.z2) * A.mu;
+                            }
+                        } else if (att_fast) {
+                            for (int j = 0; j < n_att; ++j) {
+                                const Att A = s_att[wave][j];
+                                const int k = uniform(A.k);
+                                const bool rotated = k < 0;
+                                att += crystal_chord(R, xQ, yQ, zQ, cry, rotated ? -k - 1 : k, A.x1, A.x2, A.y1, A.y2,
+                                                     A.z1, A.z2, rotated) * A.mu;
+                            }
+                        } else {
+                            for (int w = 0; w < ncw; ++w) {
+                                unsigned msk = uniform(s_mask[wave][w]);
+                                while (msk) {
+                                    const int b = __builtin_ctz(msk);
+                                    msk &= msk - 1;
+                                    const int k = uniform((int)s_cand[w * 32 + b]);
+                                    const Crystal K = cry[k];
+                                    att += crystal_chord(R, xQ, yQ, zQ, cry, k, K.cx - K.hx, K.cx + K.hx, K.cy - K.hy,
+                                                         K.cy + K.hy, K.cz - K.hz, K.cz + K.hz, K.axis == 0.0f) *
+                                           K.mu_t;
+                                }
                             }
                         }
+                        // path inside crystal d before the sub-cell, and absorption in the
+                        // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
+                        const float ux1 = SC.ux1, ux2 = SC.ux2, uy1 = SC.uy1, uy2 = SC.uy2, uz1 = SC.uz1, uz2 = SC.uz2;
+                        const QRay Rq = axis_d ? make_qray_axis(R, xPs, yPs, zPs, xDs, yDs, zDs)
+                                               : make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
+                        float a1, a2, l_abs;
+                        self_paths_centred(Rq, D.hx, D.hy, D.hz, sub_hx, sub_hy, sub_hz, ux1, ux2, uy1, uy2, uz1, uz2,
+                                           a1, a2, l_abs);
+                        att += (a1 - a2) * D.mu_t;
+                        // exp(-a) = 2^-(a log2 e): the integer part of the exponent applied exactly
+                        // (ldexp), the fraction in FP32, so the relative error stays ~2 ulp for any
+                        // attenuation (rounding a itself to FP32 would cost |a| * 6e-8)
+                        const double y2 = (att64 + (double)att) * 1.4426950408889634;
+                        const double yi = floor(y2);
+                        const float eff =
+                            (y2 < 160.0) ? ldexpf(__builtin_amdgcn_exp2f(-(float)(y2 - yi)), -(int)yi) : 0.0f;
+                        const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
+                        const float contrib = eff * solid * absorb;
+                        if (!blocked) acc += (double)contrib;
                     }
-                    // path inside crystal d before the sub-cell, and absorption in the
-                    // sub-cell along the 1000 mm extension (PESysMatGen.cu:727-759)
-                    const float ux1 = SC.ux1, ux2 = SC.ux2, uy1 = SC.uy1, uy2 = SC.uy2, uz1 = SC.uz1, uz2 = SC.uz2;
-                    const QRay Rq = axis_d ? make_qray_axis(R, xPs, yPs, zPs, xDs, yDs, zDs)
-                                           : make_qray(xPs, yPs, zPs, xDs, yDs, zDs, dist);
-                    float a1, a2, l_abs;
-                    self_paths_centred(Rq, D.hx, D.hy, D.hz, sub_hx, sub_hy, sub_hz, ux1, ux2, uy1, uy2, uz1, uz2, a1,
-                                       a2, l_abs);
-                    att += (a1 - a2) * D.mu_t;
-                    // exp(-a) = 2^-(a log2 e): the integer part of the exponent applied exactly
-                    // (ldexp), the fraction in FP32, so the relative error stays ~2 ulp for any
-                    // attenuation (rounding a itself to FP32 would cost |a| * 6e-8)
-                    const double y2 = (att64 + (double)att) * 1.4426950408889634;
-                    const double yi = floor(y2);
-                    const float eff =
-                        (y2 < 160.0) ? ldexpf(__builtin_amdgcn_exp2f(-(float)(y2 - yi)), -(int)yi) : 0.0f;
-                    const float absorb = -expm1f(-l_abs * D.mu_t) * absorb_scale;
-                    const float contrib = eff * solid * absorb;
-                    if (!blocked) acc += (double)contrib;
                 }
+                acc = wave_sum64(acc);
+                if (lane == 0) {
+                    row[(long long)izl * p.X * p.Y + iy * p.X + ix] = (float)(acc * (double)inv_nv3);
+                }
+                __builtin_amdgcn_wave_barrier();
             }
-            acc = wave_sum64(acc);
-            if (lane == 0) {
-                row[(long long)izl * p.X * p.Y + iy * p.X + ix] = (float)(acc * (double)inv_nv3);
-            }
-            __builtin_amdgcn_wave_barrier();
+            __syncthreads();  // every wave is done with this tile's lists before the next tile rewrites them
         }
-        __syncthreads();  // every wave is done with this tile's lists before the next tile rewrites them
     }
 }
 
-// Host launcher: grid = detectors x tile groups, one workgroup each.  Tile
-// groups share one cull over the detector list; a launch that would have too
-// few workgroups runs one tile per workgroup instead.  Either way every
-// element's result is the same.
+// Host launcher: grid = detectors x runs of up to kGroupsPerWorkgroup tile
+// groups, one workgroup per run.  Tile groups share one cull over the detector
+// list; a launch that would have too few workgroups runs shorter runs, then one
+// tile per workgroup.  Either way every element's result is the same.
 hipError_t launch_pe(const PeParams& p0, const Crystal* cry, const Hole* hol, const Plate* pla, const Slab* slb,
                      const PeLists& L, float* dst, int* status, hipStream_t stream) {
     PeParams p = p0;
@@ -1044,16 +1075,20 @@ hipError_t launch_pe(const PeParams& p0, const Crystal* cry, const Hole* hol, co
         p.ngy = p.nty;
         p.ngz = p.ntz;
     }
-    const long long tiles = (long long)p.ngx * p.ngy * p.ngz;
-    if (tiles <= 0 || p.det_count <= 0) return hipSuccess;
+    const long long groups = (long long)p.ngx * p.ngy * p.ngz;
+    if (groups <= 0 || p.det_count <= 0) return hipSuccess;
+    p.grp_per_wg = kGroupsPerWorkgroup;
+    while (p.grp_per_wg > 1 && (groups + p.grp_per_wg - 1) / p.grp_per_wg * p.det_count < kLoopMinWorkgroups)
+        p.grp_per_wg /= 2;
+    const long long runs = (groups + p.grp_per_wg - 1) / p.grp_per_wg;  // workgroups per detector row
     // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
     const long long max_blocks = (1LL << 32) / kThreads - 1;
-    const int chunk = (int)std::max(1LL, std::min<long long>(p.det_count, max_blocks / tiles));
+    const int chunk = (int)std::max(1LL, std::min<long long>(p.det_count, max_blocks / runs));
     for (int d0 = 0; d0 < p.det_count; d0 += chunk) {
         PeParams q = p;
         q.det_begin = p.det_begin + d0;
         q.det_count = std::min(chunk, p.det_count - d0);
-        hipLaunchKernelGGL(pe_tile_kernel, dim3((unsigned)(tiles * q.det_count)), dim3(kThreads), 0, stream, q, cry,
+        hipLaunchKernelGGL(pe_tile_kernel, dim3((unsigned)(runs * q.det_count)), dim3(kThreads), 0, stream, q, cry,
                            hol, pla, slb, L, dst + (long long)d0 * p.row_stride, status);
         const hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;

@@ -15,9 +15,14 @@ constexpr int kGroupTiles = 2;      // tiles per group along each axis
 // units, shorter tail); so do launches that would have fewer workgroups.
 constexpr long long kGroupMinTiles = 1LL << 18;
 constexpr long long kGroupMinWorkgroups = 2048;
+// A workgroup traces up to kGroupsPerWorkgroup consecutive tile groups of its
+// detector row, as long as the launch keeps at least kLoopMinWorkgroups
+// workgroups (the detector set-up and its spill stores are paid once per run).
+constexpr int kGroupsPerWorkgroup = 8;
+constexpr long long kLoopMinWorkgroups = 16384;
 // Revision of the tracer's schedule; part of the autotuner's cache key, so tile
 // choices tuned for an older schedule are not reused.
-constexpr int kPeScheduleRevision = 3;
+constexpr int kPeScheduleRevision = 4;
 
 struct PeParams {
     // image slab (in-band Img[2], Img[30], Img[31] of the reference ABI)
@@ -31,6 +36,7 @@ struct PeParams {
     // tiling
     int tx, ty, tz, ntx, nty, ntz;
     int gx, gy, gz, ngx, ngy, ngz;  // tiles per workgroup group, group counts
+    int grp_per_wg;            // consecutive tile groups per workgroup (set by launch_pe)
     // detector rows of this launch
     int n_det, det_begin, det_count;
     int n_layers, n_holes, n_slabs;
