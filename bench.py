@@ -452,6 +452,7 @@ def main():
                     help="also run a plain 256 MiB device copy (HBM counter calibration under rocprofv3)")
     ap.add_argument("--quiet", action="store_true")
     ap.add_argument("--no-autotune", action="store_true", help="use the default 4x4x4 voxel tile")
+    ap.add_argument("--tile", default="", help="voxel tile X,Y,Z for the C2 leg (no autotuning)")
     ap.add_argument("--tune-cache", default=os.environ.get("SYSMAT_TUNE_CACHE",
                                                             os.path.join(ROOT, ".tile_cache.txt")))
     args = ap.parse_args()
@@ -479,7 +480,11 @@ def main():
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
     tile, tune_info = (4, 4, 4), None
-    if not args.no_autotune:
+    if args.tile:
+        tile = tuple(int(x) for x in args.tile.split(","))
+        if len(tile) != 3 or min(tile) < 1:
+            raise SystemExit("--tile takes three positive integers X,Y,Z")
+    elif not args.no_autotune:
         os.makedirs(os.path.dirname(args.tune_cache) or ".", exist_ok=True)
         t0 = time.perf_counter()
         tile, ns_el, cached = autotune(geo, sample_rows=64, cache_path=args.tune_cache, voxel_sub=nv, sub=sub)

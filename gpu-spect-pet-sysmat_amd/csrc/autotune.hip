@@ -1,5 +1,6 @@
 // Tile-shape autotuner of the PE tracer: times candidate voxel tiles on a
-// sample of detector rows spread over the detector and keeps the fastest.
+// sample of detector rows spread over the detector and keeps the fastest, or,
+// among tiles within 2 % of it, the widest along x (longer output runs).
 // Results persist in a small text cache keyed by a hash of the geometry, the
 // image, the sampling, the GPU architecture and the tracer's schedule revision,
 // so a configuration is tuned once per machine type and kernel version.
@@ -11,6 +12,7 @@
 #include <cstdio>
 #include <cstring>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/sysmat.h"
@@ -111,13 +113,14 @@ int sm_pe_autotune(const float* col, const float* det, const float* img, const S
     DeviceBuf<float> dst;
     if ((rc = dst.alloc((size_t)per * V)) < 0) return rc;
     static const int cand[][3] = {{4, 4, 4}, {8, 4, 2}, {4, 8, 2}, {2, 4, 8}, {8, 8, 1}, {4, 4, 2},
-                                  {2, 2, 4}, {8, 8, 2}, {4, 4, 8}, {2, 8, 4}, {8, 2, 4}, {16, 4, 1}};
+                                  {2, 2, 4}, {8, 8, 2}, {4, 4, 8}, {2, 8, 4}, {8, 2, 4}, {16, 4, 1},
+                                  {16, 2, 2}, {32, 2, 1}};
+    std::vector<std::pair<double, int>> timed;  // (ms, candidate index)
     hipEvent_t e0, e1;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    double best_ms = -1.0;
-    int bt[3] = {4, 4, 4};
-    for (const auto& c : cand) {
+    for (int ci = 0; ci < (int)(sizeof(cand) / sizeof(cand[0])); ++ci) {
+        const int* c = cand[ci];
         if (c[0] > X || c[1] > Y || c[2] > zc) continue;
         SmPeOptions o = *base;
         o.tile_x = c[0];
@@ -139,15 +142,32 @@ int sm_pe_autotune(const float* col, const float* det, const float* img, const S
         if (rc >= 0) (void)hipEventElapsedTime(&ms, e0, e1);
         sm_pe_plan_destroy(plan);
         if (rc < 0) break;
-        if (best_ms < 0.0 || ms < best_ms) {
-            best_ms = ms;
-            bt[0] = c[0]; bt[1] = c[1]; bt[2] = c[2];
-        }
+        timed.push_back({(double)ms, ci});
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (rc < 0) return rc;
-    if (best_ms < 0.0) return set_error(SM_E_ARG, "no tile candidate fits the image");
+    if (timed.empty()) return set_error(SM_E_ARG, "no tile candidate fits the image");
+    // Among the tiles within kTileTieFrac of the fastest, take the widest along x:
+    // an element is one 4-byte store, so a tile's output lands in x runs of tile_x
+    // floats, and runs shorter than a cache line reach memory as partial-line
+    // writes (a 2-voxel run cost ~5.7x the output bytes in write traffic, a
+    // 32-voxel run ~1.7x, at the same tracer time).
+    constexpr double kTileTieFrac = 0.02;
+    double best_ms = timed[0].first;
+    for (const auto& tc : timed) best_ms = std::min(best_ms, tc.first);
+    int bi = -1;
+    double bi_ms = 0.0;
+    for (const auto& tc : timed) {
+        if (tc.first > best_ms * (1.0 + kTileTieFrac)) continue;
+        const int wx = cand[tc.second][0];
+        if (bi < 0 || wx > cand[bi][0] || (wx == cand[bi][0] && tc.first < bi_ms)) {
+            bi = tc.second;
+            bi_ms = tc.first;
+        }
+    }
+    best_ms = bi_ms;
+    const int bt[3] = {cand[bi][0], cand[bi][1], cand[bi][2]};
     best->tile_x = bt[0];
     best->tile_y = bt[1];
     best->tile_z = bt[2];
