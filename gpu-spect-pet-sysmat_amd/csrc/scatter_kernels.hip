@@ -545,11 +545,11 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
            (pe_abs / tot_abs);
 }
 
-// Four waves per SIMD (128 VGPRs, ~116 B/lane spilled): the per-site PE gather and
+// Four waves per SIMD (128 VGPRs, ~200 B/lane spilled): the per-site PE gather and
 // the FP64 chains of the site term need the extra waves to hide behind; measured
-// against 2 (176 VGPRs, no spill): -29 % on the C3 crystal scatter, -37 % on rows
-// of the 16384-crystal detector; 3 waves were in between, 5-8 waves spill more
-// and lose again.
+// against 2 (no or few spills): -29 % (and again -38 % in a later build) on the C3
+// crystal scatter, -37 % on rows of the 16384-crystal detector; 3 waves were in
+// between, 5-8 waves spill more and lose again.
 // kSeries: the KN logarithm by its series (photon energies up to 236 keV; launch_scatter
 // picks the instance), which keeps the library log1p and its registers out of the kernel.
 template <bool kSeries>
@@ -636,35 +636,51 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) s
             if (n_c > cap) n_c = cap;
             cap -= n_c;
             const ScPair* P = pairs + (long long)rr * p.max_sites + (long long)c * csz;
-            for (int j = 0; j < n_c; ++j) {
-                const ScPair& S = P[j];
-                // skip the site for the whole wave when every voxel's scatter angle lies outside
-                // the window band: for I within trad of the centre T the angle between S - I and
-                // S - T is at most asin(trad / |S - T|)
-                const float ax = (float)S.sx - tcx, ay = (float)S.sy - tcy, az = (float)S.sz - tcz;
-                const float la2 = ax * ax + ay * ay + az * az;
-                if (la2 > trad * trad) {
-                    const float ila = __builtin_amdgcn_rsqf(la2);
-                    const float c0 =
-                        fminf(1.0f, fmaxf(-1.0f, (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * ila));
-                    const float s1 = fminf(1.0f, trad * ila * 1.0001f);  // sin dth
-                    if (cos_test) {
-                        // th0 - dth > thh  <=>  th0 > dth and cos(th0 - dth) < cos thh;
-                        // th0 + dth < thl  <=>  th0 + dth < pi and cos(th0 + dth) > cos thl
-                        const float c1 = fsqrt(fmaxf(0.0f, 1.0f - s1 * s1));  // cos dth
-                        const float s0 = fsqrt(fmaxf(0.0f, 1.0f - c0 * c0));  // sin th0
-                        const bool above = !hi_open && c0 < c1 && c0 * c1 + s0 * s1 < chh;
-                        const bool below = !lo_open && s0 * c1 + c0 * s1 > 0.0f && c0 * c1 - s0 * s1 > cll;
-                        if (above || below) continue;
-                    } else {
-                        const float th0 = acosf(c0);
-                        const float dth = asinf(s1);
-                        if (th0 - dth > thh || th0 + dth < thl) continue;
+            for (int j0 = 0; j0 < n_c; j0 += 64) {
+                // the window-band test of 64 sites at once, one per lane: a site is dropped
+                // for the whole wave when every voxel's scatter angle lies outside the band
+                // (for I within trad of the centre T the angle between S - I and S - T is at
+                // most asin(trad / |S - T|)); the kept sites then run in index order
+                bool keep = false;
+                if (j0 + lane < n_c) {
+                    const ScPair& S = P[j0 + lane];
+                    keep = true;
+                    const float ax = (float)S.sx - tcx, ay = (float)S.sy - tcy, az = (float)S.sz - tcz;
+                    const float la2 = ax * ax + ay * ay + az * az;
+                    if (la2 > trad * trad) {
+                        const float ila = __builtin_amdgcn_rsqf(la2);
+                        const float c0 = fminf(
+                            1.0f, fmaxf(-1.0f, (ax * (float)S.bx + ay * (float)S.by + az * (float)S.bz) * ila));
+                        const float s1 = fminf(1.0f, trad * ila * 1.0001f);  // sin dth
+                        if (cos_test) {
+                            // th0 - dth > thh  <=>  th0 > dth and cos(th0 - dth) < cos thh;
+                            // th0 + dth < thl  <=>  th0 + dth < pi and cos(th0 + dth) > cos thl
+                            const float c1 = fsqrt(fmaxf(0.0f, 1.0f - s1 * s1));  // cos dth
+                            const float s0 = fsqrt(fmaxf(0.0f, 1.0f - c0 * c0));  // sin th0
+                            const bool above = !hi_open && c0 < c1 && c0 * c1 + s0 * s1 < chh;
+                            const bool below = !lo_open && s0 * c1 + c0 * s1 > 0.0f && c0 * c1 - s0 * s1 > cll;
+                            keep = !(above || below);
+                        } else {
+                            const float th0 = acosf(c0);
+                            const float dth = asinf(s1);
+                            keep = !(th0 - dth > thh || th0 + dth < thl);
+                        }
                     }
                 }
+                unsigned long long m = __ballot(keep);
                 if (!active) continue;
-                const double w = site_weight(p, R, S, q, table ? pe[(long long)S.site * p.V + v] : 0.0f);
-                acc += site_term<kSeries>(p, R, S, q, kc, w, s_kn);
+                // the PE gather of the next kept site is issued before the current site term
+                float pv = 0.0f;
+                if (table && m) pv = pe[(long long)P[j0 + __builtin_ctzll(m)].site * p.V + v];
+                while (m) {
+                    const int j = j0 + __builtin_ctzll(m);
+                    m &= m - 1;
+                    const float pv_j = pv;
+                    if (table && m) pv = pe[(long long)P[j0 + __builtin_ctzll(m)].site * p.V + v];
+                    const ScPair& S = P[j];
+                    const double w = site_weight(p, R, S, q, pv_j);
+                    acc += site_term<kSeries>(p, R, S, q, kc, w, s_kn);
+                }
             }
         }
     }
