@@ -24,6 +24,7 @@
 
 #include <algorithm>
 
+#include "approx.hpp"
 #include "kn.hpp"
 #include "scatter.hpp"
 #include "trace.hpp"
@@ -397,9 +398,12 @@ __device__ inline float window_probability(double e, double res, double lo, doub
     const double s2 = res * (1.0 / kFwhm) * e * 1.4142135623730951;
     const double is2 = 1.0 / s2;  // one division for both bounds
     const float z1 = (float)((lo - e) * is2), z2 = (float)((hi - e) * is2);
-    if (z1 >= 0.0f) return 0.5f * (erfcf(z1) - erfcf(z2));
-    if (z2 <= 0.0f) return 0.5f * (erfcf(-z2) - erfcf(-z1));
-    return 0.5f * (erff(z2) - erff(z1));
+    // both tails through erfc of |z| (branch-free; erf(z2) - erf(z1) = 2 - erfc(-z1) - erfc(z2)
+    // when z1 < 0 < z2)
+    const float a = erfc_nr(fabsf(z1)), b = erfc_nr(fabsf(z2));
+    if (z1 >= 0.0f) return 0.5f * (a - b);
+    if (z2 <= 0.0f) return 0.5f * (b - a);
+    return 0.5f * ((2.0f - a) - b);
 }
 
 // Voxel point of element v, world and crystal-r frames (scatter.cu:1153-1166),
@@ -523,10 +527,11 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
         dy = smax * cmax - cmin * sa;
         dx = cmin * cmax + smax * sa;
     }
-    // step count floor((theta_max - theta_min) / h): FP32 atan2 (error below 1e-6 rad,
-    // 1e-4 of a step) decides it unless the quotient lies within 5e-4 of a step boundary,
-    // where the FP64 atan2 does -- the same count as FP64 throughout
-    const float nf = atan2f((float)dy, (float)dx) * p.ihf;
+    // step count floor((theta_max - theta_min) / h): an FP32 atan2 (error below 3e-7 rad,
+    // 3e-5 of a step) decides it unless the quotient lies within 5e-4 of a step boundary,
+    // where the FP64 atan2 does -- the same count as FP64 throughout (dy >= 0: the angle
+    // difference lies in [0, pi])
+    const float nf = atan2_upper((float)dy, (float)dx) * p.ihf;
     const float fr = nf - floorf(nf);
     int nraw;
     if (fr > 5e-4f && fr < 1.0f - 5e-4f)
