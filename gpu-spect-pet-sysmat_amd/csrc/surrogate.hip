@@ -184,7 +184,8 @@ __device__ inline long long sg_brick(const SgParams& q, long long v) {
 // Features go through LDS into the MFMA B-operand layout; every member runs
 // with register-resident activations and weight tiles streamed from L2 (each
 // wave on its own: sharing weight slabs through LDS cost two workgroup barriers
-// per slab, more than the L2 reads it saved).  Disagreeing members flag the
+// per slab, more than the L2 reads it saved; a double-buffered form with one
+// barrier per slab needed more than 256 VGPRs for its staging and was 6 % slower).  Disagreeing members flag the
 // brick (tile) for the exact kernel, which prunes scatter sites per brick.
 // Two waves per SIMD (230 VGPRs, no spill): each output tile pair's 16 weight
 // tiles are loaded before its MFMAs (tile_pair), which needs the registers of a
