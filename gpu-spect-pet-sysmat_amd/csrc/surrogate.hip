@@ -161,12 +161,15 @@ __global__ void __launch_bounds__(256) surrogate_features_kernel(SgParams q, con
     for (int k = 0; k < kSgFeatures; ++k) x[i * kSgFeatures + k] = f[k];
 }
 
-// Voxel i (0..31) of wave chunk `ch` of a row: the run of 32 consecutive voxels
-// (row order, x fastest) from 32 ch, so each ring-prior site's PE gather of a
-// wave reads whole cache lines.
+// Voxels per wave of the apply kernel: three 16-element MFMA tiles.
+constexpr int kSgRun = 48;
+
+// Voxel i (0 .. kSgRun-1) of wave chunk `ch` of a row: the run of kSgRun consecutive
+// voxels (row order, x fastest) from kSgRun ch, so each ring-prior site's PE gather
+// of a wave reads whole cache lines.
 __device__ inline long long sg_voxel(const SgParams& q, long long ch, int i, bool& ok) {
-    const long long v = 32 * ch + i;
-    ok = v < q.V;
+    const long long v = (long long)kSgRun * ch + i;
+    ok = i < kSgRun && v < q.V;
     return v;
 }
 
@@ -180,29 +183,33 @@ __device__ inline long long sg_brick(const SgParams& q, long long v) {
     return ((long long)(iz >> 2) * nby + (iy >> 2)) * nbx + (ix >> 2);
 }
 
-// One wave = a run of 32 consecutive voxels of one detector row.
+// One wave = a run of 48 consecutive voxels of one detector row.
 // Features go through LDS into the MFMA B-operand layout; every member runs
 // with register-resident activations and weight tiles streamed from L2 (each
 // wave on its own: sharing weight slabs through LDS cost two workgroup barriers
 // per slab, more than the L2 reads it saved; a double-buffered form with one
-// barrier per slab needed more than 256 VGPRs for its staging and was 6 % slower).  Disagreeing members flag the
-// brick (tile) for the exact kernel, which prunes scatter sites per brick.
-// Two waves per SIMD (230 VGPRs, no spill): each output tile pair's 16 weight
-// tiles are loaded before its MFMAs (tile_pair), which needs the registers of a
-// third wave and hides more L2 latency than that wave did (-4 % on a 64^3 image,
-// -7 % on 128^3, identical output).
+// barrier per slab needed more than 256 VGPRs for its staging and was 6 % slower).
+// Disagreeing members flag the brick (tile) for the exact kernel, which prunes
+// scatter sites per brick.  Two waves per SIMD, no spill: three element tiles per
+// wave with each tile pair's weight tiles requested two K blocks ahead of their
+// MFMAs move 1.5x fewer weight bytes per element than two tiles with all 16
+// requested up front (MLP alone: 59.3 against 70.4 ms on 6.7e7 elements,
+// identical output; the member statistics live in LDS, which keeps the kernel at
+// 256 VGPRs without spills: 71.7 -> 61.8 ms on 256 rows of a 64^3 image); the
+// features take two rounds of 32 lanes for 48 voxels.
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) surrogate_apply_kernel(SgParams q, const SgRow* __restrict__ rows,
                                                               const ScPair* __restrict__ keep,
                                                               const int* __restrict__ keep_cnt,
                                                               const uint8_t* __restrict__ packed,
                                                               const float* __restrict__ pe, float* __restrict__ dst,
                                                               uint8_t* __restrict__ flags) {
-    __shared__ float s_f[4][32][kSgFeatures + 1];
+    __shared__ float s_f[4][kSgRun][kSgFeatures + 1];
     __shared__ float s_tab[4][kSgTab + 1];  // the wave's row angular table
     // the wave index through readfirstlane: what derives from it is known to be wave-uniform
     const int lane = threadIdx.x & 63, w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     const int g = lane >> 4, j16 = lane & 15;
-    const long long chunks = (q.V + 31) / 32;
+    constexpr int kT = kSgRun / 16;  // element tiles per wave
+    const long long chunks = (q.V + kSgRun - 1) / kSgRun;
     const long long gw = (long long)blockIdx.x * 4 + w;
     const int rr = (int)(gw / chunks);
     const long long ch = gw % chunks;
@@ -214,53 +221,77 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
         const float* tab = q.ftab + (long long)r * (kSgTab + 1);
         for (int i = lane; i <= kSgTab; i += 64) s_tab[w][i] = tab[i];
         __syncthreads();  // the rows' angular tables are in LDS
-        // lanes e and e + 32 share element e: even and odd ring sites
-        const int e = lane & 31, half = lane >> 5;
-        bool ok;
-        const long long v = sg_voxel(q, ch, e, ok);
-        const bool live = use && ok;
-        float xI = 0.0f, yI = 0.0f, zI = 0.0f, ring = 0.0f;
-        if (live) {
-            sg_position(q, v, xI, yI, zI);
-            ring = sg_ring_half(q, q.ring + 2LL * r * q.n_keep, half, q.ring_cnt[r], s_tab[w], pe, v, xI, yI, zI);
-        }
-        const float other = __shfl_xor(ring, 32, 64);
-        if (half == 0) {
-            float f[kSgFeatures];
+        // lanes e and e + 32 share element e: even and odd ring sites; two rounds
+        // (elements 0..31, then 32..47)
+        for (int rnd = 0; rnd < 2; ++rnd) {
+            const int e = 32 * rnd + (lane & 31), half = lane >> 5;
+            bool ok;
+            const long long v = sg_voxel(q, ch, e, ok);
+            const bool live = use && ok;
+            float xI = 0.0f, yI = 0.0f, zI = 0.0f, ring = 0.0f;
             if (live) {
-                sg_finish(q, R, r, v, pe, dst[rr * q.row_stride + v], ring + other, xI, yI, zI, f);
-            } else {
-#pragma unroll
-                for (int k = 0; k < kSgFeatures; ++k) f[k] = 0.0f;
+                sg_position(q, v, xI, yI, zI);
+                ring = sg_ring_half(q, q.ring + 2LL * r * q.n_keep, half, q.ring_cnt[r], s_tab[w], pe, v, xI, yI, zI);
             }
+            const float other = __shfl_xor(ring, 32, 64);
+            if (half == 0 && e < kSgRun) {
+                float f[kSgFeatures];
+                if (live) {
+                    sg_finish(q, R, r, v, pe, dst[rr * q.row_stride + v], ring + other, xI, yI, zI, f);
+                } else {
 #pragma unroll
-            for (int k = 0; k < kSgFeatures; ++k) s_f[w][e][k] = f[k];
+                    for (int k = 0; k < kSgFeatures; ++k) f[k] = 0.0f;
+                }
+#pragma unroll
+                for (int k = 0; k < kSgFeatures; ++k) s_f[w][e][k] = f[k];
+            }
         }
     }
     __syncthreads();
-    bf16x8 xf[2];
+    bf16x8 xf[kT];
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < kT; ++t)
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
             const int k = 8 * g + s;
             xf[t][s] = (bf16)(k < kSgFeatures ? s_f[w][16 * t + j16][k] : 0.0f);
         }
-    float far_sum[2] = {0.f, 0.f}, far_min[2] = {3.0e38f, 3.0e38f}, far_max[2] = {0.f, 0.f};
-    for (int m = 0; m < q.n_members; ++m) {
-        float y[2];
-        mlp_wave_forward(packed + (long long)m * kMlpPacked, xf, y);
+    // per-element member statistics (sum, min, max) in the wave's LDS feature rows, whose
+    // features are in xf now
+    float* st = &s_f[w][0][0];
+    if (g == 0) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
-            const float fr = fmaxf(__expf(y[t] * q.sig_y + q.mu_y) - q.phi, 0.0f);
-            far_sum[t] += fr;
-            far_min[t] = fminf(far_min[t], fr);
-            far_max[t] = fmaxf(far_max[t], fr);
+        for (int t = 0; t < kT; ++t) {
+            st[(16 * t + j16) * (kSgFeatures + 1) + 0] = 0.0f;
+            st[(16 * t + j16) * (kSgFeatures + 1) + 1] = 3.0e38f;
+            st[(16 * t + j16) * (kSgFeatures + 1) + 2] = 0.0f;
         }
+    }
+    for (int m = 0; m < q.n_members; ++m) {
+        float y[kT];
+        mlp_wave_forward<kT, 2>(packed + (long long)m * kMlpPacked, xf, y);
+        if (g == 0) {
+#pragma unroll
+            for (int t = 0; t < kT; ++t) {
+                const float fr = fmaxf(__expf(y[t] * q.sig_y + q.mu_y) - q.phi, 0.0f);
+                float* o = st + (16 * t + j16) * (kSgFeatures + 1);
+                o[0] += fr;
+                o[1] = fminf(o[1], fr);
+                o[2] = fmaxf(o[2], fr);
+            }
+        }
+    }
+    float far_sum[kT], far_min[kT], far_max[kT];
+#pragma unroll
+    for (int t = 0; t < kT; ++t) {
+        const float* o = st + (16 * t + j16) * (kSgFeatures + 1);
+        far_sum[t] = o[0];
+        far_min[t] = o[1];
+        far_max[t] = o[2];
     }
     if (g == 0 && use) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < kT; ++t) {
             bool ok;
             const long long v = sg_voxel(q, ch, 16 * t + j16, ok);
             if (!ok) continue;
@@ -274,7 +305,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) s
         }
     } else if (g == 0 && valid_wave) {
 #pragma unroll
-        for (int t = 0; t < 2; ++t) {
+        for (int t = 0; t < kT; ++t) {
             bool ok;
             const long long v = sg_voxel(q, ch, 16 * t + j16, ok);
             if (ok) dst[rr * q.row_stride + v] = 0.0f;
@@ -294,7 +325,7 @@ hipError_t launch_surrogate_features(const SgParams& q, const SgRow* rows, const
 
 hipError_t launch_surrogate_apply(const SgParams& q, const SgRow* rows, const ScPair* keep, const int* keep_cnt,
                                   const uint8_t* packed, const float* pe, float* dst, uint8_t* flags, hipStream_t s) {
-    const long long per_row = (q.V + 31) / 32;  // waves per detector row: runs of 32 voxels
+    const long long per_row = (q.V + kSgRun - 1) / kSgRun;  // waves per detector row: runs of 48 voxels
     if (per_row <= 0 || q.det_count <= 0) return hipSuccess;
     // gridDim.x * blockDim.x must stay below 2^32: launch detector chunks
     const long long max_waves = ((1LL << 32) / 256 - 1) * 4;
