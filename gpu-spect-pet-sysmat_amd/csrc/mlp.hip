@@ -378,16 +378,19 @@ __global__ void pack_kernel(const float* __restrict__ P, uint8_t* __restrict__ o
     }
 }
 
-// generic inference on feature rows: y[m*n + e] for members m
-__global__ void __launch_bounds__(256) mlp_infer_kernel(const uint8_t* __restrict__ packed, int n_members,
-                                                        const float* __restrict__ x, long long n,
-                                                        float* __restrict__ y) {
+// generic inference on feature rows: y[m*n + e] for members m; a wave takes 48
+// rows (three MFMA element tiles, the fused apply kernel's form; outputs are the
+// same for any tiling)
+constexpr int kInferTiles = 3;
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+mlp_infer_kernel(const uint8_t* __restrict__ packed, int n_members, const float* __restrict__ x, long long n,
+                 float* __restrict__ y) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, g = lane >> 4;
-    const long long e0 = ((long long)blockIdx.x * 4 + w) * 32;
+    const long long e0 = ((long long)blockIdx.x * 4 + w) * (16 * kInferTiles);
     if (e0 >= n) return;
-    bf16x8 xf[2];
+    bf16x8 xf[kInferTiles];
 #pragma unroll
-    for (int t = 0; t < 2; ++t) {
+    for (int t = 0; t < kInferTiles; ++t) {
         const long long e = e0 + 16 * t + (lane & 15);
 #pragma unroll
         for (int s = 0; s < 8; ++s) {
@@ -396,11 +399,11 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(const uint8_t* __restric
         }
     }
     for (int m = 0; m < n_members; ++m) {
-        float yy[2];
-        mlp_wave_forward(packed + (long long)m * kMlpPacked, xf, yy);
+        float yy[kInferTiles];
+        mlp_wave_forward<kInferTiles, 2>(packed + (long long)m * kMlpPacked, xf, yy);
         if (g == 0) {
 #pragma unroll
-            for (int t = 0; t < 2; ++t) {
+            for (int t = 0; t < kInferTiles; ++t) {
                 const long long e = e0 + 16 * t + lane;
                 if (e < n) y[(long long)m * n + e] = yy[t];
             }
@@ -411,7 +414,8 @@ __global__ void __launch_bounds__(256) mlp_infer_kernel(const uint8_t* __restric
 hipError_t launch_mlp_infer(const uint8_t* packed, int n_members, const float* x, long long n, float* y,
                             hipStream_t s) {
     if (n <= 0) return hipSuccess;
-    const long long blocks = (n + 127) / 128;
+    const long long per_block = 4 * 16 * kInferTiles;
+    const long long blocks = (n + per_block - 1) / per_block;
     hipLaunchKernelGGL(mlp_infer_kernel, dim3((unsigned)blocks), dim3(256), 0, s, packed, n_members, x, n, y);
     return hipGetLastError();
 }
