@@ -392,11 +392,29 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
 // ---------------------------------------------------------------------------
 // Per-voxel accumulation
 // ---------------------------------------------------------------------------
+// 1 / sqrt(x), x > 0: the FP64 reciprocal square root with two Newton steps (~1 ulp,
+// against the ~22 instructions of the correctly rounded sqrt and a division)
+__device__ inline double rsq_nr(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    return r;
+}
+
+// 1 / x, x != 0 and finite: the FP64 reciprocal with two Newton steps (~1 ulp, against
+// the ~10 instructions of the correctly rounded division)
+__device__ inline double rcp_nr(double x) {
+    double r = __builtin_amdgcn_rcp(x);
+    r = fma(r, fma(-x, r, 1.0), r);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return r;
+}
+
 __device__ inline float window_probability(double e, double res, double lo, double hi) {
     // 0.5 (erf(z2) - erf(z1)), z = (bound - E') / (sqrt(2) sigma); tails via erfc
     constexpr double kFwhm = (double)2.35482f;  // FWHM / sigma (reference constant)
     const double s2 = res * (1.0 / kFwhm) * e * 1.4142135623730951;
-    const double is2 = 1.0 / s2;  // one division for both bounds
+    const double is2 = rcp_nr(s2);  // one reciprocal for both bounds
     const float z1 = (float)((lo - e) * is2), z2 = (float)((hi - e) * is2);
     // both tails through erfc of |z| (branch-free; erf(z2) - erf(z1) = 2 - erfc(-z1) - erfc(z2)
     // when z1 < 0 < z2)
@@ -439,27 +457,22 @@ __device__ inline double site_weight(const ScParams& p, const ScRow& R, const Sc
     if (p.mode == 0) return (double)(pe_v * (float)(R.mu_c / R.mu_pe));
     if (p.mode == 3) return (double)pe_v;
     const double lx = q.xI - S.sx, ly = q.yI - S.sy, lz = q.zI - S.sz;
-    const double Li = sqrt(lx * lx + ly * ly + lz * lz);
+    const double iL = rsq_nr(lx * lx + ly * ly + lz * lz);  // 1 / |I - S|
+    const double iL2 = iL * iL;
     const double dyh = (double)S.hy2 - (double)S.hy1;
     double solid;
     if (p.mode == 1) {
         const double vr = (double)0.1f;
-        solid = (vr * vr * M_PI + dyh * vr) / (4.0 * M_PI * Li * Li) * fabs(S.sy - q.yI) / Li;
+        solid = (vr * vr * M_PI + dyh * vr) * (0.25 / M_PI) * iL2 * fabs(S.sy - q.yI) * iL;
     } else {
         const double vr = (double)0.2f;
-        solid = (vr * vr + dyh * vr) / (4.0 * Li * Li);
+        solid = (vr * vr + dyh * vr) * 0.25 * iL2;
     }
     return solid * p.w_col;
 }
 
-// sqrt(x), x >= 0, from the FP64 reciprocal square root with two Newton steps
-// (~1 ulp, against the ~22 instructions of the correctly rounded sqrt).
-__device__ inline double sqrt_nr(double x) {
-    double r = __builtin_amdgcn_rsq(x);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    return x > 0.0 ? x * r : 0.0;
-}
+// sqrt(x), x >= 0, as x / sqrt(x) through rsq_nr (~1 ulp).
+__device__ inline double sqrt_nr(double x) { return x > 0.0 ? x * rsq_nr(x) : 0.0; }
 
 // Contribution of one site to element (r, voxel) with source weight w
 // (scatter.cu:1240-1427 / 1736-1923).
@@ -480,9 +493,10 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
     c = fmin(1.0, fmax(-1.0, c));
     const double wc = 1.0 + ea * (1.0 - c);  // E0 / E'
-    const double es = e0 / wc;
-    // divisions by loop-invariant quantities as products (the reciprocals are hoisted)
-    const double res_s = R.res * sqrt_nr(es * (1.0 / e0));
+    // 1 / sqrt(wc) gives both E' = E0 / wc and the resolution at E', res sqrt(E' / E0)
+    const double rwc = rsq_nr(wc);
+    const double es = e0 * (rwc * rwc);
+    const double res_s = R.res * rwc;
     const double k2 = res_s * (2.0 / (double)2.35482f);
     if ((es * (1.0 + k2)) <= R.lo || (es * (1.0 - k2)) >= R.hi) return 0.0;
     const float p_e = window_probability(es, res_s, R.lo, R.hi);
@@ -500,10 +514,13 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const double dq = ((k & 1) ? qx : -qx) + ((k & 2) ? qy : -qy) + ((k & 4) ? qz : -qz) - d0;
-        const double cq = fmin(1.0, fmax(-1.0, dq * S.ib[k] * ima));
+        const double cq = dq * S.ib[k] * ima;
         cmin = fmin(cmin, cq);
         cmax = fmax(cmax, cq);
     }
+    // clamped once: min / max commute with the monotone clamp to [-1, 1]
+    cmin = fmin(1.0, fmax(-1.0, cmin));
+    cmax = fmin(1.0, fmax(-1.0, cmax));
     // theta_min = acos(cmax), theta_max = acos(cmin); theta_min = 0 when the extended
     // image -> site line passes through crystal r (scatter.cu:1298-1301, 1332-1335)
     bool through = false;
