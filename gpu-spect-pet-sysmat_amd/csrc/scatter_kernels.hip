@@ -484,12 +484,8 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     const double two_pi = 2.0 * M_PI;
     // cone angle at the site (scatter.cu:996-1015, 1246-1248)
     const double ax = S.sx - q.xI, ay = S.sy - q.yI, az = S.sz - q.zI;
-    // 1 / |S - I| from the hardware reciprocal square root and two Newton steps
-    // (relative error ~1e-16, as sqrt followed by a division)
-    const double ma2 = ax * ax + ay * ay + az * az;
-    double ima = __builtin_amdgcn_rsq(ma2);
-    ima = ima * fma(-0.5 * ma2 * ima, ima, 1.5);
-    ima = ima * fma(-0.5 * ma2 * ima, ima, 1.5);
+    // 1 / |S - I| (relative error ~1e-16, as sqrt followed by a division)
+    const double ima = rsq_nr(ax * ax + ay * ay + az * az);
     double c = (ax * S.bx + ay * S.by + az * S.bz) * ima;
     c = fmin(1.0, fmax(-1.0, c));
     const double wc = 1.0 + ea * (1.0 - c);  // E0 / E'
@@ -505,22 +501,23 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     if (S.L * sn * 2.0 <= R.rdet)
         phi = (float)two_pi;
     else
-        phi = 4.0f * asinf(fminf((float)(R.rdet * 0.5 / (S.L * sn)), 1.0f));
+        phi = 4.0f * asinf(fminf((float)(R.rdet * 0.5 * rcp_nr(S.L * sn)), 1.0f));
     // theta range over the corners of r (scatter.cu:1303-1336)
     const double asx = S.ssx - q.xIs, asy = S.ssy - q.yIs, asz = S.ssz - q.zIs;
     const double d0 = asx * S.ssx + asy * S.ssy + asz * S.ssz;
     const double qx = R.hx * asx, qy = R.hy * asy, qz = R.hz * asz;
-    double cmin = 2.0, cmax = -2.0;
+    double cmin = INFINITY, cmax = -INFINITY;
 #pragma unroll
     for (int k = 0; k < 8; ++k) {
         const double dq = ((k & 1) ? qx : -qx) + ((k & 2) ? qy : -qy) + ((k & 4) ? qz : -qz) - d0;
-        const double cq = dq * S.ib[k] * ima;
+        const double cq = dq * S.ib[k];
         cmin = fmin(cmin, cq);
         cmax = fmax(cmax, cq);
     }
-    // clamped once: min / max commute with the monotone clamp to [-1, 1]
-    cmin = fmin(1.0, fmax(-1.0, cmin));
-    cmax = fmin(1.0, fmax(-1.0, cmax));
+    // scaled by 1 / |S - I| > 0 and clamped once: min / max commute with both monotone
+    // maps, and with the same rounding as scaling each corner
+    cmin = fmin(1.0, fmax(-1.0, cmin * ima));
+    cmax = fmin(1.0, fmax(-1.0, cmax * ima));
     // theta_min = acos(cmax), theta_max = acos(cmin); theta_min = 0 when the extended
     // image -> site line passes through crystal r (scatter.cu:1298-1301, 1332-1335)
     bool through = false;
