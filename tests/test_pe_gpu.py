@@ -127,6 +127,31 @@ def test_tile_groups_bit_identical_to_single_tiles(lib):
     plan.close()
 
 
+def test_tile_group_runs_bit_identical_to_single_tiles(lib):
+    """16384 crystals with a 52^3 image: 4^3 tiles give 7 x 7 x 7 = 343 tile groups, so a
+    400-row launch (>= 16384 workgroups) runs 8 consecutive groups per workgroup with a
+    partial last run of 7; one row per launch (343 workgroups) runs one tile per
+    workgroup.  A row whose crystal is below the mu threshold takes the zero path over
+    its whole run.  Every element must be the same bit for bit."""
+    torch = pytest.importorskip("torch")
+    g = params.config5()
+    g.img[0:3] = 52
+    g.img[3:6] = 0.8
+    r0, n = 8000, 400
+    g.crystals[r0 + 17, 6] = 0.005
+    plan = pe.PePlan(g, sub=(2, 2, 2), tile=(4, 4, 4))
+    a = torch.empty((n, g.n_vox), dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    a.fill_(-1.0)
+    plan.launch(r0, n, a)
+    for i in range(n):
+        plan.launch(r0 + i, 1, b[i:])
+    plan.sync()
+    assert torch.equal(a, b) and float(a.max()) > 0.0
+    assert float(a[17].abs().max()) == 0.0 and float(a.min()) >= 0.0
+    plan.close()
+
+
 def test_bundle_filter_changes_nothing(lib):
     g = params.tiny(seed=5, n_det_side=4, n_layers=2, image=(4, 4, 4), holes_side=3, n_plates=2)
     a = pe_compute(g, bundle_filter=True)
