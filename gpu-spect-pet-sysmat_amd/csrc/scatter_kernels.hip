@@ -560,8 +560,8 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
     const float att = (float)(x3 * (double)S.A_pe + x * (double)S.A_c);
     const double pe_abs = R.mu_pe * x3, tot_abs = pe_abs + R.mu_c * x;
     const float absorb = -expm1f(-(float)((double)S.l_abs * tot_abs));
-    return w * ((double)phi / two_pi) * ratio * (double)p_e * (double)expf(-att) * (double)absorb *
-           (pe_abs / tot_abs);
+    return w * ((double)phi * (1.0 / two_pi)) * ratio * (double)p_e * (double)expf(-att) * (double)absorb *
+           (pe_abs * rcp_nr(tot_abs));
 }
 
 // Four waves per SIMD (128 VGPRs, ~200 B/lane spilled): the per-site PE gather and
