@@ -392,24 +392,6 @@ __global__ void __launch_bounds__(256) scatter_prep_kernel(ScParams p, const ScR
 // ---------------------------------------------------------------------------
 // Per-voxel accumulation
 // ---------------------------------------------------------------------------
-// 1 / sqrt(x), x > 0: the FP64 reciprocal square root with two Newton steps (~1 ulp,
-// against the ~22 instructions of the correctly rounded sqrt and a division)
-__device__ inline double rsq_nr(double x) {
-    double r = __builtin_amdgcn_rsq(x);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    r = r * fma(-0.5 * x * r, r, 1.5);
-    return r;
-}
-
-// 1 / x, x != 0 and finite: the FP64 reciprocal with two Newton steps (~1 ulp, against
-// the ~10 instructions of the correctly rounded division)
-__device__ inline double rcp_nr(double x) {
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
-    r = fma(r, fma(-x, r, 1.0), r);
-    return r;
-}
-
 __device__ inline float window_probability(double e, double res, double lo, double hi) {
     // 0.5 (erf(z2) - erf(z1)), z = (bound - E') / (sqrt(2) sigma); tails via erfc
     constexpr double kFwhm = (double)2.35482f;  // FWHM / sigma (reference constant)

@@ -25,6 +25,22 @@ constexpr float kEps = 0.001f;  // ray degeneracy / minimum chord (PESysMatGen.c
 __device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
 __device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
 
+// FP64 1/sqrt(x), x > 0, and 1/x: the hardware approximations with two Newton steps
+// (~1 ulp, against the ~22 and ~10 instructions of the correctly rounded sqrt and
+// division).  rcp_nr(+-0) is +-inf, as the division gives.
+__device__ inline double rsq_nr(double x) {
+    double r = __builtin_amdgcn_rsq(x);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    r = r * fma(-0.5 * x * r, r, 1.5);
+    return r;
+}
+__device__ inline double rcp_nr(double x) {
+    const double r0 = __builtin_amdgcn_rcp(x);
+    double r = fma(r0, fma(-x, r0, 1.0), r0);
+    r = fma(r, fma(-x, r, 1.0), r);
+    return x != 0.0 ? r : r0;
+}
+
 __device__ inline bool inside_closed(float v, float a, float b) { return v >= a && v <= b; }
 
 // A segment P -> Q prepared for repeated slab tests (world frame).
@@ -287,14 +303,15 @@ __device__ inline Ray64 make_ray64(double px, double py, double pz, double qx, d
     Ray64 r;
     r.px = px; r.py = py; r.pz = pz;
     double dx = qx - px, dy = qy - py, dz = qz - pz;
-    r.t = sqrt(dx * dx + dy * dy + dz * dz);
-    double it = 1.0 / r.t;
+    const double d2 = dx * dx + dy * dy + dz * dz;
+    const double it = rsq_nr(d2);  // 1 / |Q - P| (P != Q: the voxel lies before the collimator)
+    r.t = d2 * it;
     r.kx = dx * it; r.ky = dy * it; r.kz = dz * it;
-    r.iky = 1.0 / r.ky;
+    r.iky = rcp_nr(r.ky);
     r.sx = r.kx * r.iky;
     r.sz = r.kz * r.iky;
     r.kk = r.kx * r.kx + r.kz * r.kz;
-    r.ikk = 1.0 / r.kk;
+    r.ikk = rcp_nr(r.kk);
     return r;
 }
 
@@ -320,7 +337,7 @@ __device__ inline double chord_hole64(const Ray64& r, double xc, double y1, doub
     double disc = r.kk * rr - cr * cr;
     if (disc <= (double)0.00001f) return 0.0;
     double b = -(r.kx * ox + r.kz * oz);
-    double sq = sqrt(disc);
+    double sq = disc * rsq_nr(disc);  // disc > 1e-5
     double t1 = (b - sq) * r.ikk, t2 = (b + sq) * r.ikk;
     if (!in_l && !in_r) {
         if ((t1 >= tlo) && (t2 <= thi)) {
