@@ -8,9 +8,9 @@
   collimatorScatterSysMatCuda :1544-1936).  The bitmap here would be 20.6 GB, so both
   sides evaluate the GeometryRelationShip criterion inline.
 * Collimator-scatter geometry (2048 crystals): the GPU-built crystal bitmap
-  (537 MB, scatter.cu:1440-1540) bit-exact against the oracle's FP32 criterion on a
-  seeded sample of 384 pairs x all 2048 crystals; the collimator bitmap
-  (scatter.cu:1939-2009) bit-exact in full.
+  (537 MB, 4.3e9 bits, scatter.cu:1440-1540) and the collimator bitmap
+  (scatter.cu:1939-2009) bit-exact in full against the oracle's builders of the FP32
+  criteria.
 * Surrogate geometry (4 x 64x64 = 16384 crystals, the 51.2 mm field of view at 3.2 mm
   voxels): the surrogate trained on its own exact rows is judged on held-out rows
   against the exact kernel (total crystal-scatter sum within 1 %) and beats the same
@@ -94,16 +94,12 @@ def test_collimator_scatter_geometry_bitmaps_bit_exact(lib):
     assert n == 2048
     words = sc.bitmap_crystal(g)
     assert words.size == (n * (n + 1) // 2 * n + 31) // 32  # 537 MB
-    rng = np.random.default_rng(5)
-    i = rng.integers(0, n - 1, 384)
-    j = np.array([int(rng.integers(a + 1, n)) for a in i])
-    want = oracle.crystal_bits_for_pairs(g, np.stack([i, j], 1))
-    pair = i.astype(np.int64) * n - i.astype(np.int64) * (i + 1) // 2 + j
-    bits = pair[:, None] * n + np.arange(n, dtype=np.int64)[None, :]
-    got = ((words[bits >> 5] >> (bits & 31).astype(np.uint32)) & 1).astype(np.uint8)
-    print(f"crystal bitmap: {int(want.sum())} of {want.size} sampled bits set", flush=True)
-    assert want.sum() > 0
-    assert np.array_equal(got, want)
+    # every bit, against the oracle's builder (~10-20 s on the host's cores)
+    want = oracle.bitmap_crystal(g)
+    n_set = int(np.unpackbits(want.view(np.uint8)).sum())
+    print(f"crystal bitmap: {n_set} of {n * (n + 1) // 2 * n} bits set", flush=True)
+    assert n_set > 0
+    assert np.array_equal(words, want)
     hw = sc.bitmap_collimator(g)
     assert np.array_equal(hw, oracle.bitmap_collimator(g))
 
