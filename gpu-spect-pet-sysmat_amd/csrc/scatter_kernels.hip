@@ -546,7 +546,7 @@ __device__ inline double site_term(const ScParams& p, const ScRow& R, const ScPa
            (pe_abs * rcp_nr(tot_abs));
 }
 
-// Four waves per SIMD (128 VGPRs, ~200 B/lane spilled): the per-site PE gather and
+// Four waves per SIMD (128 VGPRs, ~160 B/lane spilled): the per-site PE gather and
 // the FP64 chains of the site term need the extra waves to hide behind; measured
 // against 2 (no or few spills): -29 % (and again -38 % in a later build) on the C3
 // crystal scatter, -37 % on rows of the 16384-crystal detector; 3 waves were in
